@@ -1,0 +1,121 @@
+"""ctypes binding of cl_forward_amd/libclf.so (the C ABI of include/clf.h).
+
+The library is required: importing the compute entry points without it
+raises, there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libclf.so")
+
+CLF_OK = 0
+MODE_PLAIN = 0
+MODE_BATCHED = 1
+
+
+class EventStats(ctypes.Structure):
+    _fields_ = [
+        ("n_tracks", ctypes.c_int32),
+        ("flags", ctypes.c_uint32),
+        ("n_tracklets", ctypes.c_int32),
+        ("n_weak", ctypes.c_int32),
+        ("max_ttf_live", ctypes.c_int32),
+        ("n_frozen", ctypes.c_int32),
+        ("h0_visits", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class LaunchConfig(ctypes.Structure):
+    _fields_ = [
+        ("block_size", ctypes.c_int32),
+        ("grid_size", ctypes.c_int32),
+        ("grain", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32),
+    ]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [
+        ("n_events", ctypes.c_int32),
+        ("stats", ctypes.POINTER(EventStats)),
+        ("track_offset", ctypes.POINTER(ctypes.c_int64)),
+        ("tracks", ctypes.POINTER(ctypes.c_uint8)),
+    ]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [("ms_run", ctypes.c_float), ("ms_search", ctypes.c_float), ("reps", ctypes.c_int32)]
+
+
+class ClfError(RuntimeError):
+    """A C-ABI call returned a status other than CLF_OK."""
+
+    def __init__(self, status: int, what: str, message: str):
+        super().__init__(f"{what}: status {status} ({message})")
+        self.status = status
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m cl_forward_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    u8p = ctypes.POINTER(ctypes.c_uint8)
+    u8pp = ctypes.POINTER(u8p)
+    sz_p = ctypes.POINTER(ctypes.c_size_t)
+    sig = {
+        "clf_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+        "clf_last_error": (ctypes.c_char_p, []),
+        "clf_version": (ctypes.c_char_p, []),
+        "clf_context_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(vp)]),
+        "clf_context_destroy": (None, [vp]),
+        "clf_context_set_config": (ctypes.c_int, [vp, ctypes.POINTER(LaunchConfig)]),
+        "clf_context_get_config": (ctypes.c_int, [vp, ctypes.POINTER(LaunchConfig)]),
+        "clf_context_num_cus": (ctypes.c_int, [vp]),
+        "clf_event_track_capacity": (ctypes.c_int64, [u8p, ctypes.c_size_t]),
+        "clf_batch_create": (ctypes.c_int, [vp, ctypes.c_int32, u8pp, sz_p, ctypes.POINTER(vp)]),
+        "clf_batch_destroy": (None, [vp]),
+        "clf_batch_run": (ctypes.c_int, [vp, vp, ctypes.c_int]),
+        "clf_batch_results": (ctypes.c_int, [vp, vp, ctypes.POINTER(Result)]),
+        "clf_batch_time": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.POINTER(Timing)]),
+        "clf_search": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int32, u8pp, sz_p,
+                                      ctypes.POINTER(Result), ctypes.POINTER(vp)]),
+        "clf_presort_event": (ctypes.c_int, [u8p, ctypes.c_size_t]),
+        "clf_fill_candidates": (ctypes.c_int, [vp, ctypes.c_int32, u8pp, sz_p,
+                                               ctypes.POINTER(ctypes.POINTER(ctypes.c_int32)),
+                                               ctypes.POINTER(ctypes.POINTER(ctypes.c_int32)),
+                                               ctypes.POINTER(ctypes.c_uint32)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+#: every C symbol include/clf.h declares
+EXPORTED = (
+    "clf_status_string", "clf_last_error", "clf_version", "clf_context_create",
+    "clf_context_destroy", "clf_context_set_config", "clf_context_get_config",
+    "clf_context_num_cus", "clf_event_track_capacity", "clf_batch_create", "clf_batch_destroy",
+    "clf_batch_run", "clf_batch_results", "clf_batch_time", "clf_search", "clf_presort_event",
+    "clf_fill_candidates",
+)
+
+
+def check(status: int, what: str):
+    if status != CLF_OK:
+        L = load()
+        raise ClfError(status, what, (L.clf_last_error() or b"").decode(errors="replace"))

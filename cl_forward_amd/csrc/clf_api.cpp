@@ -1,0 +1,580 @@
+// clf_api.cpp — host side of the C ABI (include/clf.h): contexts, resident
+// batches, launches, graphs, read-back and the host presort.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "clf.h"
+#include "clf_device.h"
+
+namespace clf {
+hipError_t launch_search(const SearchArgs& a, int block, int grid, hipStream_t stream);
+hipError_t launch_zero_i32(int32_t* p, int n, hipStream_t stream);
+hipError_t launch_pack(const EventDesc* ev, const EventResult* res, int n, int64_t* prefix,
+                       const clf_track* tracks, uint32_t* packed, hipStream_t stream);
+hipError_t occupancy(int block, int* blocks_per_cu);
+}  // namespace clf
+
+using namespace clf;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+clf_status fail(clf_status st, const std::string& msg) {
+    g_last_error = msg;
+    return st;
+}
+
+clf_status hip_fail(hipError_t e, const char* what) {
+    g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? CLF_E_OUT_OF_MEMORY : CLF_E_DEVICE;
+}
+
+#define HIP_TRY(expr)                                        \
+    do {                                                     \
+        hipError_t _e = (expr);                              \
+        if (_e != hipSuccess) return hip_fail(_e, #expr);    \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes == 0) bytes = 256;
+        if (bytes <= cap) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            p = nullptr;
+            cap = 0;
+            if (e != hipSuccess) return e;
+        }
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipSuccess) cap = bytes;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes == 0) bytes = 256;
+        if (bytes <= cap) return hipSuccess;
+        if (p) {
+            (void)hipHostFree(p);
+            p = nullptr;
+            cap = 0;
+        }
+        hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+        if (e == hipSuccess) cap = bytes;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+int32_t rd_i32(const uint8_t* p) {
+    int32_t v;
+    std::memcpy(&v, p, 4);
+    return v;
+}
+
+// Header checks of one blob; fills S and N.  Same rules as the oracle.
+bool validate_event(const uint8_t* b, size_t size, int32_t& S, int32_t& N) {
+    if (!b || size < 8) return false;
+    S = rd_i32(b);
+    N = rd_i32(b + 4);
+    if (S < 0 || S > CLF_MAX_SENSORS || N < 0 || N > CLF_MAX_EVENT_HITS) return false;
+    const size_t need = 8 + 12 * (size_t)S + 16 * (size_t)N;
+    if (size < need) return false;
+    for (int s = 0; s < S; ++s) {
+        const int32_t st = rd_i32(b + 8 + 4 * ((size_t)S + s));
+        const int32_t n = rd_i32(b + 8 + 4 * (2 * (size_t)S + s));
+        if (n < 0 || n > CLF_MAX_SENSOR_HITS || st < 0 || (int64_t)st + n > N) return false;
+    }
+    return true;
+}
+
+// Seeding and forwarding cost grows with hits on sensors two apart; used
+// only to order the work queue (largest first).
+double event_cost(const uint8_t* b, int32_t S, int32_t N) {
+    double c = N;
+    for (int s = 2; s < S; ++s) {
+        const double a = rd_i32(b + 8 + 4 * (2 * (size_t)S + s));
+        const double p = rd_i32(b + 8 + 4 * (2 * (size_t)S + s - 2));
+        c += a * p;
+    }
+    return c;
+}
+
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+}  // namespace
+
+struct clf_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n_cu = 0;
+    clf_launch_config cfg = {256, 0, 1, 1};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+};
+
+struct clf_batch {
+    clf_context* ctx = nullptr;
+    int32_t n = 0;
+    int64_t total_hits = 0, total_cap = 0;
+    size_t blob_bytes = 0;
+    DevBuf blobs, desc, order, queue, cand, used, tracks, tracklets, ttf_a, ttf_b, weak, results,
+        prefix, packed;
+    PinnedBuf h_blobs, h_results, h_prefix, h_packed;
+    std::vector<EventDesc> h_desc;
+    std::vector<int32_t> h_order;
+    // graph of one batched run, rebuilt when the launch configuration changes
+    hipGraphExec_t graph = nullptr;
+    clf_launch_config graph_cfg = {};
+    bool results_valid = false;
+};
+
+namespace {
+
+hipError_t current_device(clf_context* ctx) { return hipSetDevice(ctx->device); }
+
+int resolved_grid(clf_context* ctx, int n_items) {
+    int grid = ctx->cfg.grid_size;
+    if (grid <= 0) {
+        int per_cu = 0;
+        if (occupancy(ctx->cfg.block_size, &per_cu) != hipSuccess || per_cu <= 0) per_cu = 1;
+        grid = ctx->n_cu * per_cu;
+    }
+    return std::max(1, std::min(grid, n_items));
+}
+
+SearchArgs make_args(clf_batch* b) {
+    SearchArgs a;
+    a.blobs = b->blobs.as<uint8_t>();
+    a.ev = b->desc.as<EventDesc>();
+    a.order = b->order.as<int32_t>();
+    a.first_item = 0;
+    a.n_items = b->n;
+    a.queue_head = b->queue.as<int32_t>();
+    a.grain = 1;
+    a.cand = b->cand.as<int4>();
+    a.used = b->used.as<uint8_t>();
+    a.tracks = b->tracks.as<clf_track>();
+    a.tracklets = b->tracklets.as<uint4>();
+    a.ttf_a = b->ttf_a.as<TtfEntry>();
+    a.ttf_b = b->ttf_b.as<TtfEntry>();
+    a.weak = b->weak.as<uint32_t>();
+    a.results = b->results.as<EventResult>();
+    a.fill_only = 0;
+    return a;
+}
+
+// The kernels of one run, enqueued on the context stream.
+clf_status enqueue_run(clf_context* ctx, clf_batch* b, clf_mode mode, bool time_search) {
+    hipStream_t st = ctx->stream;
+    const int n = b->n;
+    if (n == 0) return CLF_OK;
+    // queue counters: one per launch (n+1 so plain mode has one per event)
+    HIP_TRY(launch_zero_i32(b->queue.as<int32_t>(), n + 1, st));
+    SearchArgs a = make_args(b);
+    if (time_search) HIP_TRY(hipEventRecord(ctx->ev2, st));
+    if (mode == CLF_MODE_BATCHED) {
+        HIP_TRY(launch_search(a, ctx->cfg.block_size, resolved_grid(ctx, n), st));
+    } else {
+        // plain: event i alone, in its own launch of one workgroup
+        a.order = nullptr;
+        a.n_items = 1;
+        for (int i = 0; i < n; ++i) {
+            a.first_item = i;
+            a.queue_head = b->queue.as<int32_t>() + 1 + i;
+            HIP_TRY(launch_search(a, ctx->cfg.block_size, 1, st));
+        }
+    }
+    if (time_search) HIP_TRY(hipEventRecord(ctx->ev3, st));
+    HIP_TRY(launch_pack(b->desc.as<EventDesc>(), b->results.as<EventResult>(), n,
+                        b->prefix.as<int64_t>(), b->tracks.as<clf_track>(),
+                        b->packed.as<uint32_t>(), st));
+    return CLF_OK;
+}
+
+bool same_cfg(const clf_launch_config& x, const clf_launch_config& y) {
+    return x.block_size == y.block_size && x.grid_size == y.grid_size && x.grain == y.grain;
+}
+
+clf_status run(clf_context* ctx, clf_batch* b, clf_mode mode) {
+    HIP_TRY(current_device(ctx));
+    b->results_valid = false;
+    if (mode == CLF_MODE_BATCHED && ctx->cfg.use_graph && b->n > 0) {
+        if (b->graph && !same_cfg(b->graph_cfg, ctx->cfg)) {
+            (void)hipGraphExecDestroy(b->graph);
+            b->graph = nullptr;
+        }
+        if (!b->graph) {
+            hipGraph_t g = nullptr;
+            HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            clf_status s = enqueue_run(ctx, b, mode, false);
+            hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+            if (s != CLF_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return s;
+            }
+            if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+            e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+            b->graph_cfg = ctx->cfg;
+        }
+        HIP_TRY(hipGraphLaunch(b->graph, ctx->stream));
+        return CLF_OK;
+    }
+    return enqueue_run(ctx, b, mode, false);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* clf_status_string(clf_status s) {
+    switch (s) {
+        case CLF_OK: return "ok";
+        case CLF_E_INVALID_ARGUMENT: return "invalid argument";
+        case CLF_E_DEVICE: return "device error";
+        case CLF_E_OUT_OF_MEMORY: return "out of memory";
+        case CLF_E_NO_DEVICE: return "no device";
+        case CLF_E_CAPACITY: return "capacity exceeded";
+    }
+    return "unknown status";
+}
+
+const char* clf_last_error(void) { return g_last_error.c_str(); }
+
+const char* clf_version(void) { return "clf 0.1.0 (gfx950)"; }
+
+clf_status clf_context_create(int device, clf_context** out) {
+    if (!out) return fail(CLF_E_INVALID_ARGUMENT, "out is null");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) return fail(CLF_E_NO_DEVICE, "no HIP device");
+    if (device < 0 || device >= count) return fail(CLF_E_NO_DEVICE, "device ordinal out of range");
+    HIP_TRY(hipSetDevice(device));
+    clf_context* ctx = new clf_context();
+    ctx->device = device;
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) {
+        delete ctx;
+        return hip_fail(e, "hipGetDeviceProperties");
+    }
+    ctx->n_cu = prop.multiProcessorCount;
+    e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev2);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev3);
+    if (e != hipSuccess) {
+        clf_context_destroy(ctx);
+        return hip_fail(e, "stream/event creation");
+    }
+    *out = ctx;
+    return CLF_OK;
+}
+
+void clf_context_destroy(clf_context* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
+    if (ctx->ev3) (void)hipEventDestroy(ctx->ev3);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+clf_status clf_context_set_config(clf_context* ctx, const clf_launch_config* cfg) {
+    if (!ctx || !cfg) return fail(CLF_E_INVALID_ARGUMENT, "null argument");
+    if (cfg->block_size != 64 && cfg->block_size != 128 && cfg->block_size != 256)
+        return fail(CLF_E_INVALID_ARGUMENT, "block_size must be 64, 128 or 256");
+    if (cfg->grid_size < 0) return fail(CLF_E_INVALID_ARGUMENT, "grid_size must be >= 0");
+    if (cfg->grain != 1) return fail(CLF_E_INVALID_ARGUMENT, "grain must be 1");
+    ctx->cfg = *cfg;
+    return CLF_OK;
+}
+
+clf_status clf_context_get_config(const clf_context* ctx, clf_launch_config* cfg) {
+    if (!ctx || !cfg) return fail(CLF_E_INVALID_ARGUMENT, "null argument");
+    *cfg = ctx->cfg;
+    return CLF_OK;
+}
+
+int clf_context_num_cus(const clf_context* ctx) { return ctx ? ctx->n_cu : 0; }
+
+int64_t clf_event_track_capacity(const uint8_t* event, size_t size) {
+    int32_t S, N;
+    if (!validate_event(event, size, S, N)) return -1;
+    return CLF_EVENT_TRACK_CAPACITY(N, S);
+}
+
+void clf_batch_destroy(clf_batch* b) {
+    if (!b) return;
+    if (b->ctx) {
+        (void)hipSetDevice(b->ctx->device);
+        (void)hipStreamSynchronize(b->ctx->stream);
+    }
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    DevBuf* dev[] = {&b->blobs, &b->desc, &b->order, &b->queue, &b->cand, &b->used, &b->tracks,
+                     &b->tracklets, &b->ttf_a, &b->ttf_b, &b->weak, &b->results, &b->prefix,
+                     &b->packed};
+    for (DevBuf* d : dev) d->release();
+    b->h_blobs.release();
+    b->h_results.release();
+    b->h_prefix.release();
+    b->h_packed.release();
+    delete b;
+}
+
+clf_status clf_batch_create(clf_context* ctx, int32_t n, const uint8_t* const* events,
+                            const size_t* sizes, clf_batch** out) {
+    if (!ctx || !out || n < 0 || (n > 0 && (!events || !sizes)))
+        return fail(CLF_E_INVALID_ARGUMENT, "null argument or negative count");
+    *out = nullptr;
+    HIP_TRY(current_device(ctx));
+    clf_batch* b = new clf_batch();
+    b->ctx = ctx;
+    b->n = n;
+    b->h_desc.resize(n);
+    std::vector<double> cost(n, 0.0);
+    size_t off = 0;
+    int64_t hits = 0, cap = 0;
+    for (int i = 0; i < n; ++i) {
+        int32_t S = 0, N = 0;
+        EventDesc& d = b->h_desc[i];
+        if (validate_event(events[i], sizes[i], S, N)) {
+            d.blob_off = (int64_t)off;
+            d.hit_off = hits;
+            d.track_off = cap;
+            d.n_hits = N;
+            d.n_sensors = S;
+            off += align_up(8 + 12 * (size_t)S + 16 * (size_t)N, 256);
+            hits += N;
+            cap += CLF_EVENT_TRACK_CAPACITY(N, S);
+            cost[i] = event_cost(events[i], S, N);
+        } else {
+            d.blob_off = 0;
+            d.hit_off = hits;
+            d.track_off = cap;
+            d.n_hits = -1;
+            d.n_sensors = 0;
+        }
+    }
+    b->blob_bytes = off;
+    b->total_hits = hits;
+    b->total_cap = cap;
+    b->h_order.resize(n);
+    std::iota(b->h_order.begin(), b->h_order.end(), 0);
+    std::stable_sort(b->h_order.begin(), b->h_order.end(),
+                     [&](int x, int y) { return cost[x] > cost[y]; });
+
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t r) {
+        if (e == hipSuccess) e = r;
+    };
+    chk(b->h_blobs.ensure(off));
+    chk(b->blobs.ensure(off));
+    chk(b->desc.ensure(sizeof(EventDesc) * (size_t)n));
+    chk(b->order.ensure(sizeof(int32_t) * (size_t)n));
+    chk(b->queue.ensure(sizeof(int32_t) * ((size_t)n + 1)));
+    chk(b->cand.ensure(sizeof(int4) * (size_t)hits));
+    chk(b->used.ensure((size_t)hits));
+    chk(b->tracks.ensure(sizeof(clf_track) * (size_t)cap));
+    chk(b->tracklets.ensure(sizeof(uint4) * (size_t)cap));
+    chk(b->ttf_a.ensure(sizeof(TtfEntry) * (size_t)cap));
+    chk(b->ttf_b.ensure(sizeof(TtfEntry) * (size_t)cap));
+    chk(b->weak.ensure(sizeof(uint32_t) * (size_t)cap));
+    chk(b->results.ensure(sizeof(EventResult) * (size_t)n));
+    chk(b->prefix.ensure(sizeof(int64_t) * ((size_t)n + 1)));
+    chk(b->packed.ensure(sizeof(clf_track) * (size_t)cap));
+    chk(b->h_results.ensure(sizeof(EventResult) * (size_t)n));
+    chk(b->h_prefix.ensure(sizeof(int64_t) * ((size_t)n + 1)));
+    if (e != hipSuccess) {
+        clf_batch_destroy(b);
+        return hip_fail(e, "batch allocation");
+    }
+    uint8_t* hb = b->h_blobs.as<uint8_t>();
+    for (int i = 0; i < n; ++i) {
+        const EventDesc& d = b->h_desc[i];
+        if (d.n_hits < 0) continue;
+        std::memcpy(hb + d.blob_off, events[i], 8 + 12 * (size_t)d.n_sensors + 16 * (size_t)d.n_hits);
+    }
+    hipStream_t st = ctx->stream;
+    chk(hipMemcpyAsync(b->blobs.p, hb, off, hipMemcpyHostToDevice, st));
+    chk(hipMemcpyAsync(b->desc.p, b->h_desc.data(), sizeof(EventDesc) * (size_t)n,
+                       hipMemcpyHostToDevice, st));
+    chk(hipMemcpyAsync(b->order.p, b->h_order.data(), sizeof(int32_t) * (size_t)n,
+                       hipMemcpyHostToDevice, st));
+    chk(hipStreamSynchronize(st));
+    if (e != hipSuccess) {
+        clf_batch_destroy(b);
+        return hip_fail(e, "batch upload");
+    }
+    *out = b;
+    return CLF_OK;
+}
+
+clf_status clf_batch_run(clf_context* ctx, clf_batch* b, clf_mode mode) {
+    if (!ctx || !b || b->ctx != ctx) return fail(CLF_E_INVALID_ARGUMENT, "bad context or batch");
+    if (mode != CLF_MODE_PLAIN && mode != CLF_MODE_BATCHED)
+        return fail(CLF_E_INVALID_ARGUMENT, "unknown mode");
+    return run(ctx, b, mode);
+}
+
+clf_status clf_batch_results(clf_context* ctx, clf_batch* b, clf_result* out) {
+    if (!ctx || !b || !out || b->ctx != ctx) return fail(CLF_E_INVALID_ARGUMENT, "bad argument");
+    HIP_TRY(current_device(ctx));
+    const int n = b->n;
+    hipStream_t st = ctx->stream;
+    if (n > 0) {
+        HIP_TRY(hipMemcpyAsync(b->h_results.p, b->results.p, sizeof(EventResult) * (size_t)n,
+                               hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(b->h_prefix.p, b->prefix.p, sizeof(int64_t) * ((size_t)n + 1),
+                               hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        const int64_t total = b->h_prefix.as<int64_t>()[n];
+        if (total < 0 || total > b->total_cap) return fail(CLF_E_DEVICE, "inconsistent track count");
+        HIP_TRY(b->h_packed.ensure(sizeof(clf_track) * (size_t)total));
+        if (total > 0)
+            HIP_TRY(hipMemcpyAsync(b->h_packed.p, b->packed.p, sizeof(clf_track) * (size_t)total,
+                                   hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    } else {
+        HIP_TRY(b->h_prefix.ensure(sizeof(int64_t)));
+        b->h_prefix.as<int64_t>()[0] = 0;
+    }
+    out->n_events = n;
+    out->stats = b->h_results.as<clf_event_stats>();
+    out->track_offset = b->h_prefix.as<int64_t>();
+    out->tracks = b->h_packed.p ? b->h_packed.as<uint8_t>() : nullptr;
+    b->results_valid = true;
+    return CLF_OK;
+}
+
+clf_status clf_batch_time(clf_context* ctx, clf_batch* b, clf_mode mode, int32_t warmup,
+                          int32_t reps, clf_timing* out) {
+    if (!ctx || !b || !out || b->ctx != ctx || reps <= 0 || warmup < 0)
+        return fail(CLF_E_INVALID_ARGUMENT, "bad argument");
+    HIP_TRY(current_device(ctx));
+    for (int i = 0; i < warmup; ++i) {
+        clf_status s = run(ctx, b, mode);
+        if (s != CLF_OK) return s;
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    for (int i = 0; i < reps; ++i) {
+        clf_status s = run(ctx, b, mode);
+        if (s != CLF_OK) return s;
+    }
+    HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(hipEventSynchronize(ctx->ev1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    out->ms_run = ms / (float)reps;
+    // search kernel(s) alone, without the graph, events right around them
+    float total = 0.f;
+    for (int i = 0; i < reps; ++i) {
+        clf_status s = enqueue_run(ctx, b, mode, true);
+        if (s != CLF_OK) return s;
+        HIP_TRY(hipEventSynchronize(ctx->ev3));
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, ctx->ev2, ctx->ev3));
+        total += t;
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    out->ms_search = total / (float)reps;
+    out->reps = reps;
+    b->results_valid = false;
+    return CLF_OK;
+}
+
+clf_status clf_fill_candidates(clf_context* ctx, int32_t n, const uint8_t* const* events,
+                               const size_t* sizes, int32_t* const* cand_h1,
+                               int32_t* const* cand_h2, uint32_t* flags) {
+    if (!ctx || n < 0 || (n > 0 && (!cand_h1 || !cand_h2)))
+        return fail(CLF_E_INVALID_ARGUMENT, "null argument");
+    clf_batch* b = nullptr;
+    clf_status s = clf_batch_create(ctx, n, events, sizes, &b);
+    if (s != CLF_OK) return s;
+    struct Guard {
+        clf_batch* b;
+        ~Guard() { clf_batch_destroy(b); }
+    } guard{b};
+    if (n == 0) return CLF_OK;
+    hipStream_t st = ctx->stream;
+    HIP_TRY(launch_zero_i32(b->queue.as<int32_t>(), n + 1, st));
+    SearchArgs a = make_args(b);
+    a.fill_only = 1;
+    HIP_TRY(launch_search(a, ctx->cfg.block_size, resolved_grid(ctx, n), st));
+    std::vector<int4> cand((size_t)std::max<int64_t>(b->total_hits, 1));
+    std::vector<EventResult> res(n);
+    if (b->total_hits > 0)
+        HIP_TRY(hipMemcpyAsync(cand.data(), b->cand.p, sizeof(int4) * (size_t)b->total_hits,
+                               hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(res.data(), b->results.p, sizeof(EventResult) * (size_t)n,
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int i = 0; i < n; ++i) {
+        const EventDesc& d = b->h_desc[i];
+        if (flags) flags[i] = res[i].flags;
+        if (d.n_hits < 0) continue;
+        for (int h = 0; h < d.n_hits; ++h) {
+            const int4 c = cand[(size_t)d.hit_off + h];
+            cand_h1[i][2 * h] = c.x;
+            cand_h1[i][2 * h + 1] = c.y;
+            cand_h2[i][2 * h] = c.z;
+            cand_h2[i][2 * h + 1] = c.w;
+        }
+    }
+    return CLF_OK;
+}
+
+clf_status clf_search(clf_context* ctx, clf_mode mode, int32_t n, const uint8_t* const* events,
+                      const size_t* sizes, clf_result* out, clf_batch** batch_out) {
+    if (!ctx || !out || !batch_out) return fail(CLF_E_INVALID_ARGUMENT, "null argument");
+    *batch_out = nullptr;
+    clf_batch* b = nullptr;
+    clf_status s = clf_batch_create(ctx, n, events, sizes, &b);
+    if (s != CLF_OK) return s;
+    s = clf_batch_run(ctx, b, mode);
+    if (s == CLF_OK) s = clf_batch_results(ctx, b, out);
+    if (s != CLF_OK) {
+        clf_batch_destroy(b);
+        return s;
+    }
+    *batch_out = b;
+    return CLF_OK;
+}
+
+}  // extern "C"

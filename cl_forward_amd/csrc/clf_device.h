@@ -1,0 +1,62 @@
+// clf_device.h — records shared by the HIP kernels and the host library.
+#pragma once
+
+#include <stdint.h>
+
+#include "clf_params.h"
+
+namespace clf {
+
+// Where one event of a batch lives.  Offsets are set by the host from the
+// events' headers (Kernel.cl:613-624 reads the same header on the device).
+struct EventDesc {
+    int64_t blob_off;   // byte offset of the event blob in the batch input
+    int64_t hit_off;    // first slot of the event in per-hit scratch
+    int64_t track_off;  // first record of the event in per-track scratch
+    int32_t n_hits;     // N, or -1 for a blob the host rejected
+    int32_t n_sensors;  // S
+};
+
+// Per-event outcome.  Layout is clf_event_stats of clf.h.
+struct EventResult {
+    int32_t n_tracks;
+    uint32_t flags;
+    int32_t n_tracklets;
+    int32_t n_weak;
+    int32_t max_ttf_live;
+    int32_t n_frozen;
+    int32_t h0_visits;
+    int32_t reserved;
+};
+
+// A track or tracklet to follow.  "word" is the reference's ttf word
+// (bit 31 tracklet, bits 28-30 skipped modules, bits 0-27 index,
+// Kernel.cl:222-225); the last two hits' coordinates ride along so that
+// forwarding never has to gather them again.
+struct TtfEntry {
+    uint32_t word;
+    uint32_t nhits;
+    float x0, y0, z0;  // second-to-last hit
+    float x1, y1, z1;  // last hit
+};
+
+struct SearchArgs {
+    const uint8_t* blobs;
+    const EventDesc* ev;
+    const int32_t* order;  // dequeue order of events (nullptr: identity)
+    int32_t first_item;    // items [first_item, first_item + n_items) of order
+    int32_t n_items;
+    int32_t* queue_head;   // work-queue counter, zero before the launch
+    int32_t grain;         // events taken per dequeue
+    int4* cand;            // per hit: h1 window (x,y), h2 window (z,w)
+    uint8_t* used;         // per hit
+    clf_track* tracks;     // per event, track_off records (the output)
+    uint4* tracklets;      // per event, track_off records: hits in x,y,z
+    TtfEntry* ttf_a;       // per event, track_off entries
+    TtfEntry* ttf_b;
+    uint32_t* weak;        // per event, track_off entries
+    EventResult* results;  // per event
+    int32_t fill_only;     // stop after the candidate windows (stage (a) check)
+};
+
+}  // namespace clf

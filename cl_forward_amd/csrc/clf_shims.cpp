@@ -1,0 +1,113 @@
+// clf_shims.cpp — the reference's C++ entry points (GpuPixelSearchByTriplet.h,
+// KernelInvoker.h) as thin shims over the C ABI of include/clf.h.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <mutex>
+#include <vector>
+
+#include "GpuPixelSearchByTriplet.h"
+#include "KernelInvoker.h"
+#include "clf.h"
+
+namespace {
+
+std::mutex g_mu;
+clf_context* g_ctx = nullptr;
+int g_verbosity = 0;  // 3 on the standalone path, 0 on the Gaudi path
+
+// The reference's contract for device errors: message, then exit(-1)
+// (Tools.h:27-34, KernelInvoker.cpp:56-68).
+void fatal(clf_status s, const char* what) {
+    std::cerr << "Error in function " << what << std::endl
+              << "Error string: " << clf_status_string(s) << " (" << clf_last_error() << ")"
+              << std::endl;
+    std::exit(-1);
+}
+
+clf_context* context() {
+    if (!g_ctx) {
+        clf_status s = clf_context_create(0, &g_ctx);  // DEVICE_NUMBER 0
+        if (s != CLF_OK) fatal(s, "clf_context_create");
+    }
+    return g_ctx;
+}
+
+int invoke_locked(int start, int count, const std::vector<const std::vector<uint8_t>*>& input,
+                  std::vector<std::vector<uint8_t> >& output) {
+    clf_context* ctx = context();
+    std::vector<const uint8_t*> ptrs(count);
+    std::vector<size_t> sizes(count);
+    for (int i = 0; i < count; ++i) {
+        const std::vector<uint8_t>* ev = input[start + i];
+        ptrs[i] = ev && !ev->empty() ? ev->data() : nullptr;
+        sizes[i] = ev ? ev->size() : 0;
+    }
+    if (g_verbosity >= 3) std::cout << "Invoking kernels on " << clf_version() << std::endl;
+    clf_result res;
+    clf_batch* batch = nullptr;
+    clf_status s = clf_search(ctx, CLF_MODE_BATCHED, count, ptrs.data(), sizes.data(), &res, &batch);
+    if (s != CLF_OK) fatal(s, "clf_search");
+    for (int i = 0; i < count; ++i) {
+        const int64_t a = res.track_offset[i], b = res.track_offset[i + 1];
+        std::vector<uint8_t>& out = output[start + i];
+        out.resize((size_t)(b - a) * sizeof(clf_track));
+        if (b > a) std::memcpy(out.data(), res.tracks + a * sizeof(clf_track), out.size());
+    }
+    clf_batch_destroy(batch);
+    return 0;
+}
+
+}  // namespace
+
+int invokeParallelSearch(const int startingEvent, const int eventsToProcess,
+                         const std::vector<const std::vector<uint8_t>*>& input,
+                         std::vector<std::vector<uint8_t> >& output) {
+    if (startingEvent < 0 || eventsToProcess < 0 ||
+        (size_t)startingEvent + (size_t)eventsToProcess > input.size() ||
+        (size_t)startingEvent + (size_t)eventsToProcess > output.size()) {
+        std::cerr << "invokeParallelSearch: event range out of bounds" << std::endl;
+        std::exit(-1);
+    }
+    std::lock_guard<std::mutex> lock(g_mu);
+    return invoke_locked(startingEvent, eventsToProcess, input, output);
+}
+
+int gpuPixelSearchByTripletInvocation(const std::vector<const std::vector<uint8_t>*>& input,
+                                      std::vector<std::vector<uint8_t> >& output) {
+    if (g_verbosity >= 3)
+        std::cout << "Invoking gpuPixelSearchByTriplet with " << input.size() << " events"
+                  << std::endl;
+    output.resize(input.size());
+    const int max_events_per_launch = 16000;  // GpuPixelSearchByTriplet.cpp:54
+    for (size_t i = 0; i < input.size(); i += max_events_per_launch) {
+        const size_t n = std::min(input.size() - i, (size_t)max_events_per_launch);
+        invokeParallelSearch((int)i, (int)n, input, output);
+    }
+    return 0;
+}
+
+int gpuPixelSearchByTriplet(const std::vector<const std::vector<uint8_t>*>& input,
+                            std::vector<std::vector<uint8_t> >& output) {
+    g_verbosity = 0;
+    return gpuPixelSearchByTripletInvocation(input, output);
+}
+
+int independent_execute(const std::vector<std::vector<uint8_t> >& input,
+                        std::vector<std::vector<uint8_t> >& output) {
+    std::vector<const std::vector<uint8_t>*> converted(input.size());
+    for (size_t i = 0; i < input.size(); ++i) converted[i] = &input[i];
+    g_verbosity = 3;
+    // in place through the const reference, like preorder_by_x (Tools.cpp:57-63)
+    for (size_t i = 0; i < input.size(); ++i) {
+        std::vector<uint8_t>& ev = const_cast<std::vector<uint8_t>&>(input[i]);
+        if (!ev.empty()) (void)clf_presort_event(ev.data(), ev.size());
+    }
+    return gpuPixelSearchByTripletInvocation(converted, output);
+}
+
+void independent_post_execute(const std::vector<std::vector<uint8_t> >& output) {
+    if (g_verbosity >= 3)
+        std::cout << std::endl << "Size of output: " << output.size() << " entries" << std::endl;
+}

@@ -1,0 +1,266 @@
+"""Host-side API over the HIP library, mirroring the reference's entry points.
+
+Reference names (GpuPixelSearchByTriplet.cpp:4-65, KernelInvoker.cpp:14-31):
+
+* :func:`gpu_pixel_search_by_triplet` — Gaudi path, no sort, silent
+* :func:`independent_execute` — standalone path, x-presort IN PLACE first
+  (the events must be ``bytearray`` so they can be sorted in place)
+* :func:`gpu_pixel_search_by_triplet_invocation` — common path, chunks of
+  at most 16000 events
+* :func:`invoke_parallel_search` — one chunk
+
+Each takes a list of per-event input blobs and returns/fills a list of
+per-event output blobs (``T * 100`` bytes of Track records).  Errors raise
+:class:`ClfError`.
+
+:class:`Context` and :class:`Batch` expose the resident-buffer API used by
+the benchmark and the serving loop.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+
+from . import _lib
+from ._lib import MODE_BATCHED, MODE_PLAIN, ClfError, EventStats, LaunchConfig, Result, Timing
+
+TRACK_BYTES = 100
+MAX_EVENTS_PER_LAUNCH = 16000  # GpuPixelSearchByTriplet.cpp:54
+
+__all__ = [
+    "Context", "Batch", "ClfError", "MODE_PLAIN", "MODE_BATCHED", "gpu_pixel_search_by_triplet",
+    "gpu_pixel_search_by_triplet_invocation", "independent_execute", "independent_post_execute",
+    "invoke_parallel_search", "presort_event", "track_capacity",
+]
+
+
+def _u8(buf):
+    return ctypes.cast(buf, ctypes.POINTER(ctypes.c_uint8))
+
+
+def _event_arrays(events):
+    """Keeps ctypes copies alive and returns (holders, pointer array, size array)."""
+    n = len(events)
+    holders = []
+    ptrs = (ctypes.POINTER(ctypes.c_uint8) * max(n, 1))()
+    sizes = (ctypes.c_size_t * max(n, 1))()
+    for i, ev in enumerate(events):
+        b = bytes(ev) if ev is not None else b""
+        if b:
+            h = (ctypes.c_uint8 * len(b)).from_buffer_copy(b)
+            holders.append(h)
+            ptrs[i] = _u8(h)
+        sizes[i] = len(b)
+    return holders, ptrs, sizes
+
+
+class Context:
+    """A HIP device, its stream and launch configuration (``clf_context``)."""
+
+    def __init__(self, device: int = 0, config: dict | None = None):
+        self._L = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(self._L.clf_context_create(int(device), ctypes.byref(h)), "clf_context_create")
+        self._h = h
+        if config:
+            self.set_config(**config)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.clf_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def num_cus(self) -> int:
+        return int(self._L.clf_context_num_cus(self._h))
+
+    def config(self) -> dict:
+        c = LaunchConfig()
+        _lib.check(self._L.clf_context_get_config(self._h, ctypes.byref(c)), "get_config")
+        return {k: int(getattr(c, k)) for k, _ in LaunchConfig._fields_}
+
+    def set_config(self, **kw):
+        cur = self.config()
+        cur.update({k: int(v) for k, v in kw.items()})
+        c = LaunchConfig(**cur)
+        _lib.check(self._L.clf_context_set_config(self._h, ctypes.byref(c)), "set_config")
+
+    def batch(self, events) -> "Batch":
+        return Batch(self, events)
+
+    def search(self, events, mode: int = MODE_BATCHED):
+        """Upload, run and read back.  Returns (list of output blobs, list of stats)."""
+        with self.batch(events) as b:
+            b.run(mode)
+            return b.results()
+
+    def fill_candidates(self, events):
+        """Stage (a) alone.  Returns a list of (cand_h1, cand_h2, flags) per event,
+        the windows as int32 numpy arrays of 2*N."""
+        import numpy as np
+
+        n = len(events)
+        holders, ptrs, sizes = _event_arrays(events)
+        arrs = []
+        for ev in events:
+            b = bytes(ev)
+            N = int(np.frombuffer(b[4:8], "<i4")[0]) if len(b) >= 8 else 0
+            N = max(N, 0)
+            arrs.append((np.full(2 * N + 2, -7, np.int32), np.full(2 * N + 2, -7, np.int32), N))
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        p1 = (i32p * max(n, 1))(*[a[0].ctypes.data_as(i32p) for a in arrs])
+        p2 = (i32p * max(n, 1))(*[a[1].ctypes.data_as(i32p) for a in arrs])
+        flags = (ctypes.c_uint32 * max(n, 1))()
+        _lib.check(self._L.clf_fill_candidates(self._h, n, ptrs, sizes, p1, p2, flags),
+                   "clf_fill_candidates")
+        del holders
+        return [(a[0][: 2 * a[2]], a[1][: 2 * a[2]], int(flags[i])) for i, a in enumerate(arrs)]
+
+
+class Batch:
+    """Events resident in device memory (``clf_batch``)."""
+
+    def __init__(self, ctx: Context, events):
+        self.ctx = ctx
+        self._L = ctx._L
+        self.n = len(events)
+        holders, ptrs, sizes = _event_arrays(events)
+        h = ctypes.c_void_p()
+        _lib.check(self._L.clf_batch_create(ctx.handle, self.n, ptrs, sizes, ctypes.byref(h)),
+                   "clf_batch_create")
+        del holders
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.clf_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def run(self, mode: int = MODE_BATCHED):
+        _lib.check(self._L.clf_batch_run(self.ctx.handle, self._h, int(mode)), "clf_batch_run")
+
+    def results(self):
+        r = Result()
+        _lib.check(self._L.clf_batch_results(self.ctx.handle, self._h, ctypes.byref(r)),
+                   "clf_batch_results")
+        n = r.n_events
+        offs = [int(r.track_offset[i]) for i in range(n + 1)]
+        total = offs[-1] if n else 0
+        raw = ctypes.string_at(r.tracks, total * TRACK_BYTES) if total else b""
+        outs = [raw[offs[i] * TRACK_BYTES: offs[i + 1] * TRACK_BYTES] for i in range(n)]
+        stats = []
+        for i in range(n):
+            s = r.stats[i]
+            stats.append({k: int(getattr(s, k)) for k, _ in EventStats._fields_ if k != "reserved"})
+        return outs, stats
+
+    def time(self, mode: int = MODE_BATCHED, warmup: int = 2, reps: int = 5) -> dict:
+        t = Timing()
+        _lib.check(self._L.clf_batch_time(self.ctx.handle, self._h, int(mode), int(warmup),
+                                          int(reps), ctypes.byref(t)), "clf_batch_time")
+        return {"ms_run": float(t.ms_run), "ms_search": float(t.ms_search), "reps": int(t.reps)}
+
+
+# ---------------------------------------------------------------------------
+# Reference entry points
+# ---------------------------------------------------------------------------
+_default_ctx = None
+_default_lock = threading.Lock()
+_verbosity = 0
+
+
+def _context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)  # DEVICE_NUMBER 0 (KernelDefinitions.h:7)
+    return _default_ctx
+
+
+def track_capacity(event: bytes) -> int:
+    """Upper bound on the tracks of one event; -1 for a malformed blob."""
+    L = _lib.load()
+    b = bytes(event)
+    h = (ctypes.c_uint8 * max(len(b), 1)).from_buffer_copy(b or b"\0")
+    return int(L.clf_event_track_capacity(_u8(h), len(b)))
+
+
+def presort_event(event: bytearray) -> None:
+    """Sorts one event's hits by x per sensor in place (Tools.cpp:46-66)."""
+    L = _lib.load()
+    if not isinstance(event, bytearray):
+        raise TypeError("presort_event sorts in place: pass a bytearray")
+    h = (ctypes.c_uint8 * len(event)).from_buffer(event)
+    if L.clf_presort_event(_u8(h), len(event)) != 0:
+        raise ValueError("malformed event blob (or NaN x, on which the reference's sort never ends)")
+
+
+def invoke_parallel_search(starting_event: int, events_to_process: int, input, output) -> int:
+    """KernelInvoker.cpp:14-275: fills output[starting_event : +events_to_process]."""
+    if starting_event < 0 or events_to_process < 0 or \
+            starting_event + events_to_process > min(len(input), len(output)):
+        raise ValueError("event range out of bounds")
+    with _default_lock:
+        chunk = input[starting_event: starting_event + events_to_process]
+        outs, _ = _context().search(chunk, MODE_BATCHED)
+    output[starting_event: starting_event + events_to_process] = outs
+    return 0
+
+
+def gpu_pixel_search_by_triplet_invocation(input, output: list) -> int:
+    """GpuPixelSearchByTriplet.cpp:44-65: resizes output, runs chunks of <= 16000 events."""
+    if _verbosity >= 3:
+        print(f"Invoking gpuPixelSearchByTriplet with {len(input)} events")
+    output[:] = [b""] * len(input)
+    for i in range(0, len(input), MAX_EVENTS_PER_LAUNCH):
+        invoke_parallel_search(i, min(MAX_EVENTS_PER_LAUNCH, len(input) - i), input, output)
+    return 0
+
+
+def gpu_pixel_search_by_triplet(input, output: list) -> int:
+    """GpuPixelSearchByTriplet.cpp:29-37: Gaudi path, no presort, silent."""
+    global _verbosity
+    _verbosity = 0
+    return gpu_pixel_search_by_triplet_invocation(input, output)
+
+
+def independent_execute(input, output: list) -> int:
+    """GpuPixelSearchByTriplet.cpp:4-22: presorts every event in place, then searches."""
+    global _verbosity
+    _verbosity = 3
+    for ev in input:
+        presort_event(ev)
+    return gpu_pixel_search_by_triplet_invocation(input, output)
+
+
+def independent_post_execute(output) -> None:
+    """GpuPixelSearchByTriplet.cpp:24-27."""
+    if _verbosity >= 3:
+        print(f"\nSize of output: {len(output)} entries")

@@ -1,0 +1,148 @@
+/*
+ * clf.h — C ABI of the MI355X search-by-triplet VELO pixel tracker.
+ *
+ * Plain pointers and sizes only.  Every entry point returns a clf_status;
+ * nothing here calls exit() (the reference-signature C++ shims in
+ * GpuPixelSearchByTriplet.h keep the reference's exit(-1) contract on top).
+ *
+ * Input: per-event byte blobs in the reference layout (Tools.cpp:68-80):
+ *   int32 S; int32 N; int32 sensor_Zs[S]; int32 hitStarts[S];
+ *   int32 hitNums[S]; uint32 hit_IDs[N]; float32 X[N], Y[N], Z[N]
+ * Output: per event, T records of struct Track {uint32 hitsNum;
+ * uint32 hits[24];} (KernelDefinitions.h:59-62, 100 bytes), hits being
+ * indices into the event's hit arrays, unused slots zero.
+ *
+ * Which reference interface each entry point replaces is noted on it.
+ */
+#ifndef CLF_H
+#define CLF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "clf_params.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum clf_status {
+    CLF_OK = 0,
+    CLF_E_INVALID_ARGUMENT = 1, /* bad pointer, count or configuration         */
+    CLF_E_DEVICE = 2,           /* HIP runtime or kernel launch error          */
+    CLF_E_OUT_OF_MEMORY = 3,    /* device or pinned host allocation failed     */
+    CLF_E_NO_DEVICE = 4,        /* no HIP device, or the ordinal is invalid    */
+    CLF_E_CAPACITY = 5          /* request beyond a stated maximum             */
+} clf_status;
+
+typedef enum clf_mode {
+    /* One kernel launch per event, one workgroup each: the plain path. */
+    CLF_MODE_PLAIN = 0,
+    /* One persistent launch over the whole batch: workgroups pull events
+     * from an atomic queue ordered by estimated cost. */
+    CLF_MODE_BATCHED = 1
+} clf_mode;
+
+typedef struct clf_context clf_context;
+typedef struct clf_batch clf_batch;
+
+/* Per-event outcome (mirrors the kernel's result record). */
+typedef struct clf_event_stats {
+    int32_t n_tracks;     /* output tracks (the reference's atomics[e])    */
+    uint32_t flags;       /* CLF_FLAG_* of clf_params.h                    */
+    int32_t n_tracklets;  /* tracklets made by seeding                     */
+    int32_t n_weak;       /* weak entries                                  */
+    int32_t max_ttf_live; /* largest ttf ring span over the sensor steps   */
+    int32_t n_frozen;     /* tracks frozen at 24 hits                      */
+    int32_t h0_visits;    /* h0 visited by the gather, repeats included    */
+    int32_t reserved;
+} clf_event_stats;
+
+/* Launch parameters (no effect on results). */
+typedef struct clf_launch_config {
+    int32_t block_size; /* 64, 128 or 256 threads per event workgroup      */
+    int32_t grid_size;  /* persistent workgroups; 0 = CUs x occupancy      */
+    int32_t grain;      /* events per dequeue (1)                          */
+    int32_t use_graph;  /* batched mode: capture and replay a hipGraph     */
+} clf_launch_config;
+
+/* Results of a batch; memory owned by the batch, valid until the batch
+ * runs again or is destroyed. */
+typedef struct clf_result {
+    int32_t n_events;
+    const clf_event_stats* stats; /* [n_events]                               */
+    const int64_t* track_offset;  /* [n_events + 1] record offsets in tracks  */
+    const uint8_t* tracks;        /* packed 100-byte Track records            */
+} clf_result;
+
+typedef struct clf_timing {
+    float ms_run;    /* mean wall time of one run on the device stream (all kernels) */
+    float ms_search; /* mean time of the search kernel(s) alone                      */
+    int32_t reps;
+} clf_timing;
+
+const char* clf_status_string(clf_status status);
+/* Message of the last failure on this thread ("" if none). */
+const char* clf_last_error(void);
+/* Library version string. */
+const char* clf_version(void);
+
+/* Creates a context on HIP device `device` with its own stream.
+ * Replaces the per-call OpenCL platform/context/queue/program set-up of
+ * invokeParallelSearch (KernelInvoker.cpp:30-68): nothing is compiled at
+ * run time, the gfx950 code object is part of the library. */
+clf_status clf_context_create(int device, clf_context** out);
+void clf_context_destroy(clf_context* ctx);
+clf_status clf_context_set_config(clf_context* ctx, const clf_launch_config* cfg);
+clf_status clf_context_get_config(const clf_context* ctx, clf_launch_config* cfg);
+/* Compute units of the context's device. */
+int clf_context_num_cus(const clf_context* ctx);
+
+/* Upper bound on the number of tracks of one event (CLF_EVENT_TRACK_CAPACITY),
+ * or -1 for a malformed blob. */
+int64_t clf_event_track_capacity(const uint8_t* event, size_t size);
+
+/* Uploads n events into device buffers that stay resident.  Malformed
+ * blobs are accepted and come back with CLF_FLAG_INVALID and no tracks.
+ * Replaces the buffer creation and uploads of invokeParallelSearch
+ * (KernelInvoker.cpp:77-117). */
+clf_status clf_batch_create(clf_context* ctx, int32_t n_events, const uint8_t* const* events,
+                            const size_t* sizes, clf_batch** out);
+void clf_batch_destroy(clf_batch* batch);
+/* Enqueues one search of the whole batch on the context's stream (no host
+ * synchronisation).  Replaces clEnqueueNDRangeKernel of
+ * KernelInvoker.cpp:173. */
+clf_status clf_batch_run(clf_context* ctx, clf_batch* batch, clf_mode mode);
+/* Waits for the batch and copies results to the host.  Replaces the
+ * read-back of KernelInvoker.cpp:196-206. */
+clf_status clf_batch_results(clf_context* ctx, clf_batch* batch, clf_result* out);
+/* Runs the batch warmup + reps times and reports device-side timings
+ * (hipEvents on the context's stream), like the reference's kernel-only
+ * profiling-counter timing (KernelInvoker.cpp:175-188). */
+clf_status clf_batch_time(clf_context* ctx, clf_batch* batch, clf_mode mode, int32_t warmup,
+                          int32_t reps, clf_timing* out);
+
+/* One call: upload, run, read back.  The C-ABI core of
+ * invokeParallelSearch (KernelInvoker.cpp:14-275) and of
+ * gpuPixelSearchByTripletInvocation (GpuPixelSearchByTriplet.cpp:44-65). */
+clf_status clf_search(clf_context* ctx, clf_mode mode, int32_t n_events,
+                      const uint8_t* const* events, const size_t* sizes, clf_result* out,
+                      clf_batch** batch_out);
+
+/* Stage (a) alone: the candidate windows of every hit, as fillCandidates
+ * leaves dev_hit_candidates and dev_hit_h2_candidates (Kernel.cl:50-178,
+ * -1 where it writes nothing).  cand_h1[i] and cand_h2[i] hold 2*N_i int32
+ * for event i; flags (nullable) receives the input-only CLF_FLAG_* bits. */
+clf_status clf_fill_candidates(clf_context* ctx, int32_t n_events, const uint8_t* const* events,
+                               const size_t* sizes, int32_t* const* cand_h1,
+                               int32_t* const* cand_h2, uint32_t* flags);
+
+/* x-presort of one event in place, per sensor, with the reference's
+ * partition sequence so that tied x keep its order (preorder_by_x,
+ * Tools.cpp:46-66, 117-164).  Host code; 0 or -1 for a malformed blob. */
+int clf_presort_event(uint8_t* event, size_t size);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CLF_H */

@@ -1,0 +1,148 @@
+"""HIP path against the oracle on the same inputs (bit-exact).
+
+Every comparison goes through the C ABI of libclf.so; the oracle is the
+checker only."""
+from __future__ import annotations
+
+import struct
+import sys
+
+import numpy as np
+import pytest
+
+from cl_forward_amd import generator
+from cl_forward_amd.search import MODE_BATCHED, MODE_PLAIN
+from tests.parity import assert_event_equal, canonical, describe_diff
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _check_windows(ctx, oracle, blobs, tag):
+    got = ctx.fill_candidates(blobs)
+    for i, (b, (g1, g2, flags)) in enumerate(zip(blobs, got)):
+        o1, o2 = oracle.fill_candidates(b)
+        if not (np.array_equal(g1, o1) and np.array_equal(g2, o2)):
+            bad1 = np.nonzero(g1 != o1)[0][:5]
+            bad2 = np.nonzero(g2 != o2)[0][:5]
+            raise AssertionError(f"{tag}[{i}]: windows differ at h1 {bad1.tolist()} "
+                                 f"{g1[bad1].tolist()} vs {o1[bad1].tolist()}, h2 {bad2.tolist()} "
+                                 f"{g2[bad2].tolist()} vs {o2[bad2].tolist()}")
+        assert flags == oracle.input_flags(b), f"{tag}[{i}] input flags"
+
+
+def _check_search(ctx, oracle, blobs, tag, mode=MODE_BATCHED):
+    outs, stats = ctx.search(blobs, mode)
+    for i, b in enumerate(blobs):
+        o_out, o_st = oracle.search_event(b)
+        assert_event_equal(outs[i], stats[i], o_out, o_st, f"{tag}[{i}]")
+    return outs, stats
+
+
+def test_fill_candidates_mcdata_sorted(gpu_ctx, oracle_mod, mcdata_sorted):
+    _say("fill_candidates: 52 mcdata events, x-sorted")
+    _check_windows(gpu_ctx, oracle_mod, list(mcdata_sorted.values()), "mcdata")
+
+
+def test_fill_candidates_mcdata_unsorted(gpu_ctx, oracle_mod, mcdata_raw):
+    _say("fill_candidates: 52 mcdata events as stored (unsorted: exact scans)")
+    _check_windows(gpu_ctx, oracle_mod, list(mcdata_raw.values()), "mcdata-raw")
+
+
+def test_fill_candidates_synthetic(gpu_ctx, oracle_mod, single_blob, plain51_blobs):
+    _say("fill_candidates: single + 51 plain events")
+    _check_windows(gpu_ctx, oracle_mod, [single_blob] + plain51_blobs, "synthetic")
+
+
+def test_search_single(gpu_ctx, oracle_mod, single_blob):
+    _say("search: single event (150 tracks, seed 1), batched and plain")
+    for mode in (MODE_BATCHED, MODE_PLAIN):
+        _check_search(gpu_ctx, oracle_mod, [single_blob], f"single/mode{mode}", mode)
+
+
+def test_search_mcdata(gpu_ctx, oracle_mod, mcdata_sorted, oracle_digests):
+    _say("search: 52 mcdata events (reference-presorted), batched")
+    outs, stats = _check_search(gpu_ctx, oracle_mod, list(mcdata_sorted.values()), "mcdata")
+    import hashlib
+
+    for (stem, _), out in zip(mcdata_sorted.items(), outs):
+        assert hashlib.sha256(out).hexdigest() == oracle_digests["mcdata"][stem]["sha256"], stem
+
+
+def test_search_mcdata_unsorted(gpu_ctx, oracle_mod, mcdata_raw):
+    _say("search: 52 mcdata events unsorted (Gaudi path without presort)")
+    _check_search(gpu_ctx, oracle_mod, list(mcdata_raw.values()), "mcdata-raw")
+
+
+def test_search_plain51_both_modes(gpu_ctx, oracle_mod, plain51_blobs):
+    _say("search: 51 plain events, plain mode and batched mode")
+    a, _ = _check_search(gpu_ctx, oracle_mod, plain51_blobs, "plain51/plain", MODE_PLAIN)
+    b, _ = _check_search(gpu_ctx, oracle_mod, plain51_blobs, "plain51/batched", MODE_BATCHED)
+    assert a == b
+
+
+@pytest.mark.parametrize("block", [64, 128, 256])
+def test_block_sizes_identical(oracle_mod, plain51_blobs, block):
+    from cl_forward_amd.search import Context
+
+    _say(f"search: block size {block}")
+    with Context(0) as ctx:
+        ctx.set_config(block_size=block)
+        _check_search(ctx, oracle_mod, plain51_blobs[:12], f"block{block}")
+
+
+def test_determinism_and_graph_replays(gpu_ctx, plain51_blobs):
+    _say("search: repeated graph replays give identical bytes")
+    with gpu_ctx.batch(plain51_blobs) as b:
+        b.run(MODE_BATCHED)
+        first, _ = b.results()
+        for r in range(4):
+            # host work between replays
+            _ = sum(len(x) for x in first)
+            b.run(MODE_BATCHED)
+            again, _ = b.results()
+            assert again == first, f"replay {r}"
+        b.run(MODE_PLAIN)
+        plain, _ = b.results()
+        assert plain == first
+
+
+def test_stress_events(gpu_ctx, oracle_mod):
+    _say("search: 6 high-occupancy events (sensors above 128 hits)")
+    blobs = [generator.stress_event(i).pack() for i in range(6)]
+    _check_search(gpu_ctx, oracle_mod, blobs, "stress")
+
+
+def _blob(S, N, Z, starts, nums, x, y, z, ids=None):
+    ids = np.arange(N, dtype=np.uint32) if ids is None else ids
+    return (struct.pack("<ii", S, N) + np.asarray(Z, "<i4").tobytes() +
+            np.asarray(starts, "<i4").tobytes() + np.asarray(nums, "<i4").tobytes() +
+            np.asarray(ids, "<u4").tobytes() + np.asarray(x, "<f4").tobytes() +
+            np.asarray(y, "<f4").tobytes() + np.asarray(z, "<f4").tobytes())
+
+
+def test_edge_cases(gpu_ctx, oracle_mod, single_blob):
+    _say("search: empty, tiny, malformed and sensor-sparse events")
+    from cl_forward_amd.event import parse_event
+
+    empty = _blob(0, 0, [], [], [], [], [], [])
+    ev = parse_event(single_blob)
+    # only the first 4 sensors: the sensor loop never seeds
+    n4 = int(ev.hitNums[:4].sum())
+    four = _blob(4, n4, ev.sensor_Zs[:4], ev.hitStarts[:4], ev.hitNums[:4], ev.X[:n4], ev.Y[:n4],
+                 ev.Z[:n4], ev.hit_IDs[:n4])
+    # every other sensor emptied
+    nums = ev.hitNums.copy()
+    nums[::2] = 0
+    sparse = _blob(ev.n_sensors, ev.n_hits, ev.sensor_Zs, ev.hitStarts, nums, ev.X, ev.Y, ev.Z, ev.hit_IDs)
+    truncated = single_blob[:-4]
+    blobs = [empty, four, sparse, truncated, single_blob]
+    outs, stats = gpu_ctx.search(blobs, MODE_BATCHED)
+    for i in (0, 1, 2, 4):
+        o_out, o_st = oracle_mod.search_event(blobs[i])
+        assert_event_equal(outs[i], stats[i], o_out, o_st, f"edge[{i}]")
+    assert outs[3] == b"" and stats[3]["flags"] & (1 << 7), "truncated blob must come back INVALID"
