@@ -82,6 +82,7 @@ def load():
         "clf_context_set_config": (ctypes.c_int, [vp, ctypes.POINTER(LaunchConfig)]),
         "clf_context_get_config": (ctypes.c_int, [vp, ctypes.POINTER(LaunchConfig)]),
         "clf_context_num_cus": (ctypes.c_int, [vp]),
+        "clf_context_synchronize": (ctypes.c_int, [vp]),
         "clf_event_track_capacity": (ctypes.c_int64, [u8p, ctypes.c_size_t]),
         "clf_batch_create": (ctypes.c_int, [vp, ctypes.c_int32, u8pp, sz_p, ctypes.POINTER(vp)]),
         "clf_batch_destroy": (None, [vp]),
@@ -92,6 +93,8 @@ def load():
         "clf_search": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int32, u8pp, sz_p,
                                       ctypes.POINTER(Result), ctypes.POINTER(vp)]),
         "clf_presort_event": (ctypes.c_int, [u8p, ctypes.c_size_t]),
+        "clf_probe_copy": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.c_int32,
+                                          ctypes.POINTER(ctypes.c_float)]),
         "clf_fill_candidates": (ctypes.c_int, [vp, ctypes.c_int32, u8pp, sz_p,
                                                ctypes.POINTER(ctypes.POINTER(ctypes.c_int32)),
                                                ctypes.POINTER(ctypes.POINTER(ctypes.c_int32)),
@@ -109,9 +112,9 @@ def load():
 EXPORTED = (
     "clf_status_string", "clf_last_error", "clf_version", "clf_context_create",
     "clf_context_destroy", "clf_context_set_config", "clf_context_get_config",
-    "clf_context_num_cus", "clf_event_track_capacity", "clf_batch_create", "clf_batch_destroy",
+    "clf_context_num_cus", "clf_context_synchronize", "clf_event_track_capacity", "clf_batch_create", "clf_batch_destroy",
     "clf_batch_run", "clf_batch_results", "clf_batch_time", "clf_search", "clf_presort_event",
-    "clf_fill_candidates",
+    "clf_fill_candidates", "clf_probe_copy",
 )
 
 

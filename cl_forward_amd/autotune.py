@@ -1,0 +1,100 @@
+"""Autotuner of the launch parameters of the batched path.
+
+Sweeps the workgroup size {64, 128, 256} and the persistent grid (a
+multiple of the compute-unit count, or the occupancy-derived default) on a
+resident batch, checks that every configuration returns the same bytes,
+and records the fastest in ``cl_forward_amd/tuned.json``:
+
+    python -m cl_forward_amd.autotune [--workload batch1000] [--reps 5]
+
+Launch parameters never change results (the parity tests run every block
+size); the tuner verifies that again for the batch it times.
+"""
+from __future__ import annotations
+
+import argparse
+import hashlib
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TUNED_PATH = os.path.join(HERE, "tuned.json")
+
+BLOCKS = (64, 128, 256)
+GRID_PER_CU = (0, 1, 2, 4, 8)  # 0 = occupancy-derived default
+
+
+def load_tuned(num_cus: int, workload: str | None = None) -> dict | None:
+    """Best configuration recorded for a device with this many CUs."""
+    try:
+        with open(TUNED_PATH) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        return None
+    entries = [e for e in data.get("entries", []) if e.get("num_cus") == num_cus]
+    if workload:
+        same = [e for e in entries if e.get("workload") == workload]
+        entries = same or entries
+    return dict(entries[0]["best"]) if entries else None
+
+
+def tune(ctx, batch, reps: int = 5, warmup: int = 1, log=print) -> dict:
+    from .search import MODE_BATCHED
+
+    table = []
+    ref_digest = None
+    for block in BLOCKS:
+        for per_cu in GRID_PER_CU:
+            grid = per_cu * ctx.num_cus
+            ctx.set_config(block_size=block, grid_size=grid, use_graph=1)
+            t = batch.time(MODE_BATCHED, warmup=warmup, reps=reps)
+            batch.run(MODE_BATCHED)
+            outs, _ = batch.results()
+            digest = hashlib.sha256(b"".join(outs)).hexdigest()
+            if ref_digest is None:
+                ref_digest = digest
+            same = digest == ref_digest
+            row = {"block_size": block, "grid_size": grid, "ms_run": round(t["ms_run"], 4),
+                   "ms_search": round(t["ms_search"], 4), "identical": same}
+            table.append(row)
+            log(json.dumps(row), flush=True) if log is print else log(row)
+            if not same:
+                raise RuntimeError(f"launch configuration {row} changed the results")
+    best = min(table, key=lambda r: r["ms_run"])
+    return {"best": {"block_size": best["block_size"], "grid_size": best["grid_size"],
+                     "grain": 1, "use_graph": 1},
+            "table": table}
+
+
+def main(argv=None):
+    from . import generator
+    from .search import Context
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="batch1000")
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--out", default=TUNED_PATH)
+    a = ap.parse_args(argv)
+    blobs = generator.workload_blobs(a.workload)
+    with Context(0) as ctx, ctx.batch(blobs) as b:
+        t0 = time.time()
+        res = tune(ctx, b, reps=a.reps)
+        entry = {"num_cus": ctx.num_cus, "workload": a.workload, "tuned_at": int(time.time()),
+                 "seconds": round(time.time() - t0, 1), **res}
+    try:
+        with open(a.out) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        data = {"entries": []}
+    data["entries"] = [e for e in data.get("entries", [])
+                       if not (e.get("num_cus") == entry["num_cus"] and e.get("workload") == a.workload)]
+    data["entries"].insert(0, entry)
+    with open(a.out, "w") as f:
+        json.dump(data, f, indent=1)
+    print(json.dumps(entry["best"]))
+
+
+if __name__ == "__main__":
+    sys.exit(main())

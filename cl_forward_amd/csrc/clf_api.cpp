@@ -18,6 +18,7 @@ hipError_t launch_zero_i32(int32_t* p, int n, hipStream_t stream);
 hipError_t launch_pack(const EventDesc* ev, const EventResult* res, int n, int64_t* prefix,
                        const clf_track* tracks, uint32_t* packed, hipStream_t stream);
 hipError_t occupancy(int block, int* blocks_per_cu);
+hipError_t launch_probe_copy(const void* src, void* dst, int64_t bytes, hipStream_t stream);
 }  // namespace clf
 
 using namespace clf;
@@ -333,6 +334,13 @@ clf_status clf_context_get_config(const clf_context* ctx, clf_launch_config* cfg
 
 int clf_context_num_cus(const clf_context* ctx) { return ctx ? ctx->n_cu : 0; }
 
+clf_status clf_context_synchronize(clf_context* ctx) {
+    if (!ctx) return fail(CLF_E_INVALID_ARGUMENT, "null context");
+    HIP_TRY(current_device(ctx));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return CLF_OK;
+}
+
 int64_t clf_event_track_capacity(const uint8_t* event, size_t size) {
     int32_t S, N;
     if (!validate_event(event, size, S, N)) return -1;
@@ -516,6 +524,32 @@ clf_status clf_batch_time(clf_context* ctx, clf_batch* b, clf_mode mode, int32_t
     out->ms_search = total / (float)reps;
     out->reps = reps;
     b->results_valid = false;
+    return CLF_OK;
+}
+
+clf_status clf_probe_copy(clf_context* ctx, int64_t bytes, int32_t reps, float* ms_per_copy) {
+    if (!ctx || bytes < 16 || reps < 1) return fail(CLF_E_INVALID_ARGUMENT, "bad argument");
+    HIP_TRY(current_device(ctx));
+    DevBuf src, dst;
+    struct Guard {
+        DevBuf& s;
+        DevBuf& d;
+        ~Guard() {
+            s.release();
+            d.release();
+        }
+    } guard{src, dst};
+    HIP_TRY(src.ensure((size_t)bytes));
+    HIP_TRY(dst.ensure((size_t)bytes));
+    HIP_TRY(launch_zero_i32(src.as<int32_t>(), (int)std::min<int64_t>(bytes / 4, 1 << 30), ctx->stream));
+    HIP_TRY(launch_probe_copy(src.p, dst.p, bytes, ctx->stream));  // warm-up
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    for (int i = 0; i < reps; ++i) HIP_TRY(launch_probe_copy(src.p, dst.p, bytes, ctx->stream));
+    HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(hipEventSynchronize(ctx->ev1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (ms_per_copy) *ms_per_copy = ms / (float)reps;
     return CLF_OK;
 }
 

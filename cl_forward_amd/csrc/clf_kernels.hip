@@ -911,9 +911,27 @@ __global__ __launch_bounds__(256) void pack_tracks_kernel(const EventDesc* ev,
     for (int64_t w = threadIdx.x; w < n_words; w += blockDim.x) dst[w] = src[w];
 }
 
+// Plain 16-byte-per-lane copy: the known-size reference kernel against
+// which the HBM counters (FETCH_SIZE, WRITE_SIZE) of the search are scaled.
+__global__ __launch_bounds__(256) void probe_copy_kernel(const uint4* __restrict__ src,
+                                                         uint4* __restrict__ dst, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------
 // Launchers (called from the host library)
 // ---------------------------------------------------------------------------
+hipError_t launch_probe_copy(const void* src, void* dst, int64_t bytes, hipStream_t stream) {
+    const int64_t n = bytes / 16;
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 256 * 32);
+    hipLaunchKernelGGL(probe_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, stream,
+                       (const uint4*)src, (uint4*)dst, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_search(const SearchArgs& a, int block, int grid, hipStream_t stream) {
     switch (block) {
         case 64:

@@ -86,6 +86,9 @@ class Context:
     def __exit__(self, *exc):
         self.close()
 
+    def synchronize(self):
+        _lib.check(self._L.clf_context_synchronize(self._h), "clf_context_synchronize")
+
     @property
     def num_cus(self) -> int:
         return int(self._L.clf_context_num_cus(self._h))
@@ -100,6 +103,13 @@ class Context:
         cur.update({k: int(v) for k, v in kw.items()})
         c = LaunchConfig(**cur)
         _lib.check(self._L.clf_context_set_config(self._h, ctypes.byref(c)), "set_config")
+
+    def probe_copy(self, nbytes: int, reps: int = 10) -> float:
+        """Mean ms of the calibration copy kernel over nbytes (read + write)."""
+        ms = ctypes.c_float()
+        _lib.check(self._L.clf_probe_copy(self._h, int(nbytes), int(reps), ctypes.byref(ms)),
+                   "clf_probe_copy")
+        return float(ms.value)
 
     def batch(self, events) -> "Batch":
         return Batch(self, events)

@@ -97,6 +97,9 @@ clf_status clf_context_set_config(clf_context* ctx, const clf_launch_config* cfg
 clf_status clf_context_get_config(const clf_context* ctx, clf_launch_config* cfg);
 /* Compute units of the context's device. */
 int clf_context_num_cus(const clf_context* ctx);
+/* Blocks until all work enqueued on the context's stream has finished
+ * (the reference's clFinish / clWaitForEvents, KernelInvoker.cpp:169,175). */
+clf_status clf_context_synchronize(clf_context* ctx);
 
 /* Upper bound on the number of tracks of one event (CLF_EVENT_TRACK_CAPACITY),
  * or -1 for a malformed blob. */
@@ -136,6 +139,12 @@ clf_status clf_search(clf_context* ctx, clf_mode mode, int32_t n_events,
 clf_status clf_fill_candidates(clf_context* ctx, int32_t n_events, const uint8_t* const* events,
                                const size_t* sizes, int32_t* const* cand_h1,
                                int32_t* const* cand_h2, uint32_t* flags);
+
+/* Measurement probe, no counterpart in the reference: copies `bytes`
+ * (a multiple of 16) between two scratch buffers with a plain 16-byte-per-lane
+ * kernel, `reps` times, and reports the mean duration.  Its known traffic
+ * (bytes read + bytes written) calibrates HBM counters. */
+clf_status clf_probe_copy(clf_context* ctx, int64_t bytes, int32_t reps, float* ms_per_copy);
 
 /* x-presort of one event in place, per sensor, with the reference's
  * partition sequence so that tied x keep its order (preorder_by_x,
