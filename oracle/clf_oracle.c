@@ -445,47 +445,76 @@ static void seed_one(state_t* s, int h0, int f) {
     if (ulist_push(&s->ttf, CLF_TTF_TRACKLET_BIT | (uint32_t)p)) s->oom = 1;
 }
 
-/* The h0 gather of one sensor step (Kernel.cl:720-754) followed by
- * trackCreation for each chunk.  sh_hit_lastPointer restarts from 0 each
- * step (:684-686) and keeps counting across chunks. */
+/* The h0 gather (Kernel.cl:720-754).  sh_hit_lastPointer restarts from 0
+ * each sensor step (:684-686) and keeps counting across the chunks of the
+ * step.  Chunk c reads elements prev+lane; a lane whose element is used
+ * takes values v from the shared counter and looks at prev + shift + v until
+ * it finds an unused or out-of-range element.  The r-th such lane in lane
+ * order takes the r-th of those elements.  After the chunk prev = counter +
+ * shift and shift grows by 64 (:753-754). */
+typedef struct {
+    uint32_t prev, shift, counter;
+} gather_t;
+
+static void gather_init(gather_t* g) {
+    g->prev = 0;
+    g->shift = CLF_GATHER_CHUNK;
+    g->counter = 0;
+}
+
+/* Fills lane_e[0..63] with the element (0..hn0-1) each lane of the next chunk
+ * processes, or -1.  Returns 0 when the step has no chunk left. */
+static int gather_next_chunk(gather_t* g, uint32_t hn0, const uint8_t* used,
+                             int lane_e[CLF_GATHER_CHUNK]) {
+    if (!(g->prev < hn0)) return 0;
+    int need[CLF_GATHER_CHUNK], n_need = 0;
+    for (int x = 0; x < CLF_GATHER_CHUNK; ++x) {
+        const uint32_t e = g->prev + (uint32_t)x;
+        lane_e[x] = -1;
+        if (e < hn0) {
+            if (!used[e]) lane_e[x] = (int)e;
+            else need[n_need++] = x;
+        }
+    }
+    for (int r = 0; r < n_need; ++r) {
+        for (;;) {
+            const uint32_t e = g->prev + g->shift + g->counter++;
+            if (!(e < hn0)) break;
+            if (!used[e]) {
+                lane_e[need[r]] = (int)e;
+                break;
+            }
+        }
+    }
+    g->prev = g->counter + g->shift;
+    g->shift += CLF_GATHER_CHUNK;
+    return 1;
+}
+
+int clf_oracle_gather(int hn0, const uint8_t* used, int32_t* visits) {
+    gather_t g;
+    gather_init(&g);
+    int lane_e[CLF_GATHER_CHUNK];
+    int nv = 0;
+    while (gather_next_chunk(&g, (uint32_t)(hn0 < 0 ? 0 : hn0), used, lane_e))
+        for (int x = 0; x < CLF_GATHER_CHUNK; ++x)
+            if (lane_e[x] >= 0) visits[nv++] = lane_e[x];
+    return nv;
+}
+
+/* One sensor step of seeding: the gather, then trackCreation for each
+ * chunk's h0 in lane order. */
 static void seed_step(state_t* s, int f) {
     const event_t* ev = s->ev;
     const int start0 = ev->start[f];
-    const uint32_t hn0 = (uint32_t)ev->num[f];
-    uint32_t prev = 0;                       /* sh_hit_prevPointer */
-    uint32_t shift = CLF_GATHER_CHUNK;       /* shift_lastPointer  */
-    uint32_t counter = 0;                    /* sh_hit_lastPointer */
-    while (prev < hn0) {
-        int lane_h0[CLF_GATHER_CHUNK];
-        int need[CLF_GATHER_CHUNK], n_need = 0;
+    gather_t g;
+    gather_init(&g);
+    int lane_e[CLF_GATHER_CHUNK];
+    while (gather_next_chunk(&g, (uint32_t)ev->num[f], s->used + start0, lane_e)) {
         for (int x = 0; x < CLF_GATHER_CHUNK; ++x) {
-            const uint32_t e = prev + (uint32_t)x;
-            lane_h0[x] = -1;
-            if (e < hn0) {
-                if (!s->used[start0 + (int)e])
-                    lane_h0[x] = start0 + (int)e;
-                else
-                    need[n_need++] = x;
-            }
-        }
-        /* The r-th lane needing a replacement takes values from the shared
-         * counter until it sees an unused or out-of-range element. */
-        for (int r = 0; r < n_need; ++r) {
-            for (;;) {
-                const uint32_t e = prev + shift + counter++;
-                if (!(e < hn0)) break;
-                if (!s->used[start0 + (int)e]) {
-                    lane_h0[need[r]] = start0 + (int)e;
-                    break;
-                }
-            }
-        }
-        prev = counter + shift;
-        shift += CLF_GATHER_CHUNK;
-        for (int x = 0; x < CLF_GATHER_CHUNK; ++x) {
-            if (lane_h0[x] < 0) continue;
+            if (lane_e[x] < 0) continue;
             s->st->seed_h0_visits++;
-            seed_one(s, lane_h0[x], f);
+            seed_one(s, start0 + lane_e[x], f);
         }
     }
 }

@@ -61,6 +61,13 @@ int clf_oracle_search_batch(int n, const uint8_t* const* blobs,
  * the same partition sequence, so tied x keep the reference's order. */
 int clf_oracle_presort(uint8_t* blob, size_t size);
 
+/* The h0 gather of one sensor step alone (Kernel.cl:720-754): hn0 hits,
+ * used[0..hn0) their flags.  Writes the visited element indices in the
+ * restatement's lane order to visits (room for hn0 + 64) and returns their
+ * number; with more than 128 hits some unused hits are skipped and some
+ * visited twice, as in the reference. */
+int clf_oracle_gather(int hn0, const uint8_t* used, int32_t* visits);
+
 /* Per-event status flags computable from the input alone
  * (CLF_FLAG_DEGENERATE_Z, CLF_FLAG_UNSORTED, CLF_FLAG_INVALID). */
 uint32_t clf_oracle_input_flags(const uint8_t* blob, size_t size);

@@ -73,6 +73,8 @@ def lib():
         L.clf_oracle_presort.restype = ctypes.c_int
         L.clf_oracle_input_flags.argtypes = [u8p, ctypes.c_size_t]
         L.clf_oracle_input_flags.restype = ctypes.c_uint32
+        L.clf_oracle_gather.argtypes = [ctypes.c_int, u8p, ctypes.POINTER(ctypes.c_int32)]
+        L.clf_oracle_gather.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -143,6 +145,17 @@ def search_batch(blobs, n_threads=1, keep_output=True):
     if not keep_output:
         return None, sts
     return [bytes(outs[i])[: sts[i]["n_tracks"] * TRACK_BYTES] for i in range(n)], sts
+
+
+def gather(used) -> list[int]:
+    """Visited element indices of one sensor step's h0 gather, in lane order."""
+    u = np.ascontiguousarray(np.asarray(used, dtype=np.uint8))
+    n = int(u.size)
+    out = np.empty(n + 64, np.int32)
+    nv = lib().clf_oracle_gather(n, u.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)) if n else
+                                 ctypes.cast((ctypes.c_uint8 * 1)(), ctypes.POINTER(ctypes.c_uint8)),
+                                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    return out[:nv].tolist()
 
 
 def presort(blob: bytes) -> bytes:
