@@ -108,6 +108,8 @@ def main(argv=None):
                     help="launch the kernels directly instead of replaying a graph")
     ap.add_argument("--probe-copy", type=int, default=0, metavar="BYTES",
                     help="only run the calibration copy kernel over BYTES and exit")
+    ap.add_argument("--profile-phases", action="store_true",
+                    help="only print the per-phase time breakdown of one plain and one batched run")
     a = ap.parse_args(argv)
 
     if a.probe_copy:
@@ -148,6 +150,27 @@ def main(argv=None):
     cfg = ctx.config()
     batch = ctx.batch(blobs)
     log(f"uploaded; config {cfg}; {ctx.num_cus} CUs")
+
+    if a.profile_phases:
+        import numpy as np
+
+        report = {"workload": a.workload, "config": cfg}
+        for mode, name in ((MODE_PLAIN, "plain"), (MODE_BATCHED, "batched")):
+            p = batch.profile(mode)
+            tot = p["total"]
+            phases = {k: {"mean_us": round(float(np.mean(p[k])) / 1e3, 2),
+                          "share": round(float(np.sum(p[k]) / max(np.sum(tot), 1)), 4)}
+                      for k in batch.PROFILE_FIELDS[:9]}
+            counts = {k: round(float(np.mean(p[k])), 1) for k in batch.PROFILE_FIELDS[10:15]}
+            slow = int(np.argmax(tot))
+            report[name] = {"event_us_mean": round(float(np.mean(tot)) / 1e3, 2),
+                            "event_us_max": round(float(np.max(tot)) / 1e3, 2),
+                            "phases": phases, "counts_mean": counts,
+                            "slowest_event": {k: round(float(p[k][slow]) / (1e3 if i < 10 else 1), 2)
+                                              for i, k in enumerate(batch.PROFILE_FIELDS[:15])}}
+            log(f"{name}: {json.dumps(report[name])}")
+        print(json.dumps(report), flush=True)
+        return 0
 
     # results of the batched path (and the plain path: identical tracks)
     batch.run(MODE_BATCHED)

@@ -35,6 +35,7 @@ class LaunchConfig(ctypes.Structure):
         ("grid_size", ctypes.c_int32),
         ("grain", ctypes.c_int32),
         ("use_graph", ctypes.c_int32),
+        ("lds_hits", ctypes.c_int32),
     ]
 
 
@@ -95,6 +96,8 @@ def load():
         "clf_presort_event": (ctypes.c_int, [u8p, ctypes.c_size_t]),
         "clf_probe_copy": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.c_int32,
                                           ctypes.POINTER(ctypes.c_float)]),
+        "clf_batch_profile": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
+                                             ctypes.POINTER(ctypes.c_double)]),
         "clf_fill_candidates": (ctypes.c_int, [vp, ctypes.c_int32, u8pp, sz_p,
                                                ctypes.POINTER(ctypes.POINTER(ctypes.c_int32)),
                                                ctypes.POINTER(ctypes.POINTER(ctypes.c_int32)),
@@ -114,7 +117,7 @@ EXPORTED = (
     "clf_context_destroy", "clf_context_set_config", "clf_context_get_config",
     "clf_context_num_cus", "clf_context_synchronize", "clf_event_track_capacity", "clf_batch_create", "clf_batch_destroy",
     "clf_batch_run", "clf_batch_results", "clf_batch_time", "clf_search", "clf_presort_event",
-    "clf_fill_candidates", "clf_probe_copy",
+    "clf_fill_candidates", "clf_probe_copy", "clf_batch_profile",
 )
 
 

@@ -1,8 +1,8 @@
 """Autotuner of the launch parameters of the batched path.
 
-Sweeps the workgroup size {64, 128, 256} and the persistent grid (a
-multiple of the compute-unit count, or the occupancy-derived default) on a
-resident batch, checks that every configuration returns the same bytes,
+Sweeps the workgroup size {64, 128, 256, 512}, the hits per LDS sensor slot
+{256, 512} and the persistent grid (a multiple of the compute-unit count, or
+the occupancy-derived default) on a resident batch, checks that every configuration returns the same bytes,
 and records the fastest in ``cl_forward_amd/tuned.json``:
 
     python -m cl_forward_amd.autotune [--workload batch1000] [--reps 5]
@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import argparse
 import hashlib
+import itertools
 import json
 import os
 import sys
@@ -22,8 +23,9 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 TUNED_PATH = os.path.join(HERE, "tuned.json")
 
-BLOCKS = (64, 128, 256)
-GRID_PER_CU = (0, 1, 2, 4, 8)  # 0 = occupancy-derived default
+BLOCKS = (64, 128, 256, 512)
+LDS_HITS = (256, 512)
+GRID_PER_CU = (0, 1, 2, 4)  # 0 = occupancy-derived default
 
 
 def load_tuned(num_cus: int, workload: str | None = None) -> dict | None:
@@ -45,26 +47,26 @@ def tune(ctx, batch, reps: int = 5, warmup: int = 1, log=print) -> dict:
 
     table = []
     ref_digest = None
-    for block in BLOCKS:
-        for per_cu in GRID_PER_CU:
-            grid = per_cu * ctx.num_cus
-            ctx.set_config(block_size=block, grid_size=grid, use_graph=1)
-            t = batch.time(MODE_BATCHED, warmup=warmup, reps=reps)
-            batch.run(MODE_BATCHED)
-            outs, _ = batch.results()
-            digest = hashlib.sha256(b"".join(outs)).hexdigest()
-            if ref_digest is None:
-                ref_digest = digest
-            same = digest == ref_digest
-            row = {"block_size": block, "grid_size": grid, "ms_run": round(t["ms_run"], 4),
-                   "ms_search": round(t["ms_search"], 4), "identical": same}
-            table.append(row)
-            log(json.dumps(row), flush=True) if log is print else log(row)
-            if not same:
-                raise RuntimeError(f"launch configuration {row} changed the results")
+    for block, lds_hits, per_cu in itertools.product(BLOCKS, LDS_HITS, GRID_PER_CU):
+        grid = per_cu * ctx.num_cus
+        ctx.set_config(block_size=block, grid_size=grid, use_graph=1, lds_hits=lds_hits)
+        t = batch.time(MODE_BATCHED, warmup=warmup, reps=reps)
+        batch.run(MODE_BATCHED)
+        outs, _ = batch.results()
+        digest = hashlib.sha256(b"".join(outs)).hexdigest()
+        if ref_digest is None:
+            ref_digest = digest
+        same = digest == ref_digest
+        row = {"block_size": block, "lds_hits": lds_hits, "grid_size": grid,
+               "ms_run": round(t["ms_run"], 4), "ms_search": round(t["ms_search"], 4),
+               "identical": same}
+        table.append(row)
+        log(json.dumps(row), flush=True) if log is print else log(row)
+        if not same:
+            raise RuntimeError(f"launch configuration {row} changed the results")
     best = min(table, key=lambda r: r["ms_run"])
     return {"best": {"block_size": best["block_size"], "grid_size": best["grid_size"],
-                     "grain": 1, "use_graph": 1},
+                     "lds_hits": best["lds_hits"], "grain": 1, "use_graph": 1},
             "table": table}
 
 

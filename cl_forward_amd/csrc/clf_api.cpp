@@ -13,11 +13,12 @@
 #include "clf_device.h"
 
 namespace clf {
-hipError_t launch_search(const SearchArgs& a, int block, int grid, hipStream_t stream);
+hipError_t launch_search(const SearchArgs& a, int block, int lds_hits, int grid, hipStream_t stream);
+bool config_supported(int block, int lds_hits);
 hipError_t launch_zero_i32(int32_t* p, int n, hipStream_t stream);
 hipError_t launch_pack(const EventDesc* ev, const EventResult* res, int n, int64_t* prefix,
                        const clf_track* tracks, uint32_t* packed, hipStream_t stream);
-hipError_t occupancy(int block, int* blocks_per_cu);
+hipError_t occupancy(int block, int lds_hits, int* blocks_per_cu);
 hipError_t launch_probe_copy(const void* src, void* dst, int64_t bytes, hipStream_t stream);
 }  // namespace clf
 
@@ -138,7 +139,7 @@ struct clf_context {
     int device = 0;
     hipStream_t stream = nullptr;
     int n_cu = 0;
-    clf_launch_config cfg = {256, 0, 1, 1};
+    clf_launch_config cfg = {256, 0, 1, 1, 256};
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
 };
 
@@ -166,7 +167,8 @@ int resolved_grid(clf_context* ctx, int n_items) {
     int grid = ctx->cfg.grid_size;
     if (grid <= 0) {
         int per_cu = 0;
-        if (occupancy(ctx->cfg.block_size, &per_cu) != hipSuccess || per_cu <= 0) per_cu = 1;
+        if (occupancy(ctx->cfg.block_size, ctx->cfg.lds_hits, &per_cu) != hipSuccess || per_cu <= 0)
+            per_cu = 1;
         grid = ctx->n_cu * per_cu;
     }
     return std::max(1, std::min(grid, n_items));
@@ -190,6 +192,7 @@ SearchArgs make_args(clf_batch* b) {
     a.weak = b->weak.as<uint32_t>();
     a.results = b->results.as<EventResult>();
     a.fill_only = 0;
+    a.prof = nullptr;
     return a;
 }
 
@@ -203,7 +206,7 @@ clf_status enqueue_run(clf_context* ctx, clf_batch* b, clf_mode mode, bool time_
     SearchArgs a = make_args(b);
     if (time_search) HIP_TRY(hipEventRecord(ctx->ev2, st));
     if (mode == CLF_MODE_BATCHED) {
-        HIP_TRY(launch_search(a, ctx->cfg.block_size, resolved_grid(ctx, n), st));
+        HIP_TRY(launch_search(a, ctx->cfg.block_size, ctx->cfg.lds_hits, resolved_grid(ctx, n), st));
     } else {
         // plain: event i alone, in its own launch of one workgroup
         a.order = nullptr;
@@ -211,7 +214,7 @@ clf_status enqueue_run(clf_context* ctx, clf_batch* b, clf_mode mode, bool time_
         for (int i = 0; i < n; ++i) {
             a.first_item = i;
             a.queue_head = b->queue.as<int32_t>() + 1 + i;
-            HIP_TRY(launch_search(a, ctx->cfg.block_size, 1, st));
+            HIP_TRY(launch_search(a, ctx->cfg.block_size, ctx->cfg.lds_hits, 1, st));
         }
     }
     if (time_search) HIP_TRY(hipEventRecord(ctx->ev3, st));
@@ -222,7 +225,8 @@ clf_status enqueue_run(clf_context* ctx, clf_batch* b, clf_mode mode, bool time_
 }
 
 bool same_cfg(const clf_launch_config& x, const clf_launch_config& y) {
-    return x.block_size == y.block_size && x.grid_size == y.grid_size && x.grain == y.grain;
+    return x.block_size == y.block_size && x.grid_size == y.grid_size && x.grain == y.grain &&
+           x.lds_hits == y.lds_hits;
 }
 
 clf_status run(clf_context* ctx, clf_batch* b, clf_mode mode) {
@@ -318,8 +322,9 @@ void clf_context_destroy(clf_context* ctx) {
 
 clf_status clf_context_set_config(clf_context* ctx, const clf_launch_config* cfg) {
     if (!ctx || !cfg) return fail(CLF_E_INVALID_ARGUMENT, "null argument");
-    if (cfg->block_size != 64 && cfg->block_size != 128 && cfg->block_size != 256)
-        return fail(CLF_E_INVALID_ARGUMENT, "block_size must be 64, 128 or 256");
+    if (!config_supported(cfg->block_size, cfg->lds_hits))
+        return fail(CLF_E_INVALID_ARGUMENT,
+                    "unsupported (block_size, lds_hits): block 64/128/256/512, lds_hits 256/512");
     if (cfg->grid_size < 0) return fail(CLF_E_INVALID_ARGUMENT, "grid_size must be >= 0");
     if (cfg->grain != 1) return fail(CLF_E_INVALID_ARGUMENT, "grain must be 1");
     ctx->cfg = *cfg;
@@ -527,6 +532,46 @@ clf_status clf_batch_time(clf_context* ctx, clf_batch* b, clf_mode mode, int32_t
     return CLF_OK;
 }
 
+clf_status clf_batch_profile(clf_context* ctx, clf_batch* b, clf_mode mode, uint64_t* out,
+                             double* tick_ns) {
+    if (!ctx || !b || b->ctx != ctx || (b->n > 0 && !out))
+        return fail(CLF_E_INVALID_ARGUMENT, "bad argument");
+    if (mode != CLF_MODE_PLAIN && mode != CLF_MODE_BATCHED)
+        return fail(CLF_E_INVALID_ARGUMENT, "unknown mode");
+    HIP_TRY(current_device(ctx));
+    const int n = b->n;
+    int rate_khz = 0;
+    HIP_TRY(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device));
+    if (tick_ns) *tick_ns = rate_khz > 0 ? 1.0e6 / (double)rate_khz : 0.0;
+    if (n == 0) return CLF_OK;
+    DevBuf prof;
+    HIP_TRY(prof.ensure(sizeof(unsigned long long) * (size_t)n * kProfSlots));
+    struct Guard {
+        DevBuf& p;
+        ~Guard() { p.release(); }
+    } guard{prof};
+    hipStream_t st = ctx->stream;
+    HIP_TRY(launch_zero_i32(b->queue.as<int32_t>(), n + 1, st));
+    SearchArgs a = make_args(b);
+    a.prof = prof.as<unsigned long long>();
+    if (mode == CLF_MODE_BATCHED) {
+        HIP_TRY(launch_search(a, ctx->cfg.block_size, ctx->cfg.lds_hits, resolved_grid(ctx, n), st));
+    } else {
+        a.order = nullptr;
+        a.n_items = 1;
+        for (int i = 0; i < n; ++i) {
+            a.first_item = i;
+            a.queue_head = b->queue.as<int32_t>() + 1 + i;
+            HIP_TRY(launch_search(a, ctx->cfg.block_size, ctx->cfg.lds_hits, 1, st));
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(out, prof.p, sizeof(unsigned long long) * (size_t)n * kProfSlots,
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    b->results_valid = false;
+    return CLF_OK;
+}
+
 clf_status clf_probe_copy(clf_context* ctx, int64_t bytes, int32_t reps, float* ms_per_copy) {
     if (!ctx || bytes < 16 || reps < 1) return fail(CLF_E_INVALID_ARGUMENT, "bad argument");
     HIP_TRY(current_device(ctx));
@@ -570,7 +615,7 @@ clf_status clf_fill_candidates(clf_context* ctx, int32_t n, const uint8_t* const
     HIP_TRY(launch_zero_i32(b->queue.as<int32_t>(), n + 1, st));
     SearchArgs a = make_args(b);
     a.fill_only = 1;
-    HIP_TRY(launch_search(a, ctx->cfg.block_size, resolved_grid(ctx, n), st));
+    HIP_TRY(launch_search(a, ctx->cfg.block_size, ctx->cfg.lds_hits, resolved_grid(ctx, n), st));
     std::vector<int4> cand((size_t)std::max<int64_t>(b->total_hits, 1));
     std::vector<EventResult> res(n);
     if (b->total_hits > 0)

@@ -57,6 +57,11 @@ struct SearchArgs {
     uint32_t* weak;        // per event, track_off entries
     EventResult* results;  // per event
     int32_t fill_only;     // stop after the candidate windows (stage (a) check)
+    // Optional per-event phase profile (nullptr: off): kProfSlots counters per
+    // event, see the PROF_* indices in clf_kernels.hip.
+    unsigned long long* prof;
 };
+
+constexpr int kProfSlots = 16;
 
 }  // namespace clf

@@ -2,31 +2,36 @@
 //
 // One workgroup owns one event at a time and pulls events from an atomic
 // work queue (persistent kernel).  Inside an event the sensor steps
-// f = S-1 .. 4 run in sequence (the reference's Kernel.cl:671-767 chain);
-// every step is forwarding, then the h0 gather, then seeding, each spread
-// over the workgroup.  The stages:
+// f = S-1 .. 4 run in sequence (the reference's Kernel.cl:671-767 chain):
+// each step is forwarding, then the h0 gather, then seeding, each spread
+// over the workgroup.
 //
-//   (a) fill_candidates   Kernel.cl:50-178   h1 / h2 windows per hit, by
-//                                             binary search on x-sorted
-//                                             sensors, exact scans otherwise
-//   (c) forwarding        Kernel.cl:199-368  best h2 on sensor f-4 over the
-//                                             x-range a 0.6 mm tolerance
-//                                             allows (binary search)
-//   (b) gather + seeding  Kernel.cl:392-568, 720-754
-//                                             the reference's local size
-//                                             (64,4) semantics: 64-wide h0
-//                                             chunks, 4 rows of h1 with the
-//                                             per-row early stop
-//   (d) used flags, weak  Kernel.cl:313-363, 769-815
+// Data placement.  A step touches three sensors: f (h0), f-2 (h1) and f-4
+// (h2, and the forwarding target).  Their hit coordinates live in a ring of
+// five LDS sensor slots (f .. f-4): every step stages one new sensor, f-4,
+// with coalesced loads.  A sensor with more hits than a slot holds (C, a
+// template parameter the autotuner picks) is read from HBM instead through
+// the same generic pointers.  The used flags are an LDS bitmask.  Only the
+// ttf lists, tracklets and output tracks go to HBM.
 //
-// Every append (tracks, tracklets, ttf entries, weak entries) goes through
-// a wave ballot / prefix sum in a fixed order instead of a global atomic,
-// so the output — order included — is the same on every run.
+// Stages:
+//   (a) candidate windows  Kernel.cl:50-178   window_h1 / window_h2: binary
+//       search on x-sorted sensor slices in LDS, exact scans otherwise;
+//       h2 windows of sensor f-2 at every step, h1 windows per gathered h0
+//   (c) forwarding         Kernel.cl:199-368  best h2 on sensor f-4 over the
+//       x-range a 0.6 mm tolerance allows
+//   (b) gather + seeding   Kernel.cl:392-568, 720-754  the (64,4) semantics:
+//       64-wide h0 chunks, 4 rows of h1 with the per-row stop
+//   (d) used flags, weak   Kernel.cl:313-363, 769-815
 //
-// FP32, compiled with -ffp-contract=off: each expression keeps the
-// reference's operation order.  The restriction of a scan to an x-range
-// is exact: it drops only hits whose |dx| >= 0.6 mm is proven from the
-// monotonicity of IEEE rounding (see x_window()).
+// Every append (tracks, tracklets, ttf entries, weak entries) goes through a
+// wave ballot / prefix sum in a fixed order instead of a global atomic, so
+// the output, order included, is the same on every run.
+//
+// FP32, built with -ffp-contract=off: every expression keeps the reference's
+// operation order.  Restricting a scan to an x-range is exact: it drops only
+// hits whose |dx| >= 0.6 mm follows from the monotonicity of IEEE rounding
+// (x_window()).
 #include <hip/hip_runtime.h>
 
 #include <float.h>
@@ -38,8 +43,36 @@
 namespace clf {
 
 constexpr int kWave = 64;
-constexpr int kMaxVisits = 256;  // gathered h0 handled per seeding batch
-constexpr float kLim = 1.0e5f;   // |coordinate| bound of the supported domain
+constexpr int kMaxVisits = 128;   // gathered h0 handled per seeding batch (multiple of 64)
+constexpr int kRing = 5;          // LDS sensor slots: f .. f-4
+constexpr int kUsedBits = 32768;  // events up to this many hits keep used flags in LDS
+constexpr float kLim = 1.0e5f;    // |coordinate| bound of the supported domain
+
+// Phase profile slots (wall-clock ticks, then work counters), filled when
+// SearchArgs::prof is set.  Thread 0 takes a timestamp after the barriers
+// that close each phase.
+enum ProfSlot {
+    PROF_PROLOGUE = 0,   // header, per-sensor checks, ring set-up
+    PROF_STAGE = 1,      // staging sensor f-4 into the ring
+    PROF_FORWARD = 2,    // h2 windows of sensor f-2 + forwarding
+    PROF_GATHER = 3,     // the h0 gather (wave 0)
+    PROF_SEED_SETUP = 4, // h1 windows of the gathered h0, pair prefix
+    PROF_SEED_STOP = 5,  // row stop points
+    PROF_SEED_EVAL = 6,  // (h0, h1) pairs: h2 range and fits
+    PROF_SEED_EMIT = 7,  // tracklets + ttf entries
+    PROF_TAIL = 8,       // tail + weak filter
+    PROF_TOTAL = 9,
+    CNT_FWD_ENTRIES = 10,
+    CNT_VISITS = 11,
+    CNT_PAIRS = 12,
+    CNT_BATCHES = 13,
+    CNT_FWD_TILES = 14,
+};
+
+struct PhaseClock {
+    bool on;
+    unsigned long long t;
+};
 
 // ---------------------------------------------------------------------------
 // Wave / block helpers
@@ -76,16 +109,23 @@ __device__ __forceinline__ int wave_incl_scan(int x) {
     return x;
 }
 
-template <int B>
+template <int B, int C>
 struct Shared {
     static constexpr int W = B / kWave;
+    // per-sensor data of the current event
     int32_t s_start[CLF_MAX_SENSORS];
     int32_t s_num[CLF_MAX_SENSORS];
     int32_t s_Z[CLF_MAX_SENSORS];
     float s_zmin[CLF_MAX_SENSORS];
     float s_zmax[CLF_MAX_SENSORS];
     int32_t s_sorted[CLF_MAX_SENSORS];
-    int32_t s_pre[CLF_MAX_SENSORS + 1];
+    // ring of sensor slots: hit coordinates of sensors f .. f-4
+    float rx[kRing][C];
+    float ry[kRing][C];
+    float rz[kRing][C];
+    // h2 windows of the hits of sensor f-2, relative to sensor f-4's start
+    int2 w2[C];
+    uint32_t used_bits[kUsedBits / 32];
     int32_t scan[3][W];
     int32_t item;
     uint32_t flags;
@@ -94,21 +134,36 @@ struct Shared {
     // gather state of the current sensor step (Kernel.cl:720-721,753-754)
     uint32_t g_prev, g_shift, g_counter;
     int32_t g_nv, g_more;
-    // the current batch of gathered h0
+    // the current batch of gathered h0 (indices relative to sensor f)
     int32_t v_h0[kMaxVisits];
-    int32_t v_first[kMaxVisits];
+    int32_t v_first[kMaxVisits];  // h1 window start, relative to sensor f-2
     int32_t v_nh[kMaxVisits];
     int32_t v_pre[kMaxVisits + 1];
     float v_x0[kMaxVisits], v_y0[kMaxVisits], v_z0[kMaxVisits], v_dymax[kMaxVisits];
     int32_t v_death[kMaxVisits * CLF_SEED_ROWS];
     unsigned long long v_best[kMaxVisits];
+    unsigned long long prof[kProfSlots];
 };
+
+template <int B, int C>
+__device__ __forceinline__ void prof_mark(Shared<B, C>& sh, PhaseClock& pc, int slot) {
+    if (pc.on && threadIdx.x == 0) {
+        const unsigned long long now = wall_clock64();
+        sh.prof[slot] += now - pc.t;
+        pc.t = now;
+    }
+}
+template <int B, int C>
+__device__ __forceinline__ void prof_add(Shared<B, C>& sh, const PhaseClock& pc, int slot,
+                                         unsigned long long v) {
+    if (pc.on && threadIdx.x == 0) sh.prof[slot] += v;
+}
 
 // Exclusive prefix counts of up to three predicates over the block, in
 // thread order.  Every thread of the block must call it.
-template <int B>
-__device__ __forceinline__ void block_scan3(Shared<B>& sh, bool p0, bool p1, bool p2, int (&pos)[3],
-                                            int (&tot)[3]) {
+template <int B, int C>
+__device__ __forceinline__ void block_scan3(Shared<B, C>& sh, bool p0, bool p1, bool p2,
+                                            int (&pos)[3], int (&tot)[3]) {
     const int lane = lane_id(), wave = wave_id();
     const uint64_t m[3] = {ballot(p0), ballot(p1), ballot(p2)};
 #pragma unroll
@@ -120,7 +175,7 @@ __device__ __forceinline__ void block_scan3(Shared<B>& sh, bool p0, bool p1, boo
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         int base = 0, t = 0;
-        for (int w = 0; w < Shared<B>::W; ++w) {
+        for (int w = 0; w < Shared<B, C>::W; ++w) {
             const int c = sh.scan[k][w];
             base += (w < wave) ? c : 0;
             t += c;
@@ -132,7 +187,7 @@ __device__ __forceinline__ void block_scan3(Shared<B>& sh, bool p0, bool p1, boo
 }
 
 // ---------------------------------------------------------------------------
-// Searches over an x-sorted slice
+// Searches over an x-sorted slice (generic pointers: LDS or HBM)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int lb_ge(const float* x, int lo, int hi, float v) {
     while (lo < hi) {
@@ -170,10 +225,10 @@ __device__ __forceinline__ int first_diff_ge(const float* x, int lo, int hi, flo
 }
 
 // Widened bounds [lo, hi] of the x of hits that can pass |x_pred - x| < 0.6.
-// x_pred over the sensor is bracketed by pa and pb, its values at the
-// sensor's extreme z (the prediction chain is monotone in z under IEEE
-// rounding).  If x > pmax + 0.6 exactly, fl(x_pred - x) <= -0.6f, so the
-// cut fails; the margin covers the rounding of the bound itself.
+// x_pred over a sensor is bracketed by pa and pb, its values at the sensor's
+// extreme z (the prediction chain is monotone in z under IEEE rounding).  If
+// x > pmax + 0.6 exactly, fl(x_pred - x) <= -0.6f and the cut fails; the
+// margin covers the rounding of the bound itself.
 __device__ __forceinline__ void x_window(float pa, float pb, float& lo, float& hi) {
     lo = fminf(pa, pb) - CLF_PARAM_TOLERANCE;
     lo = lo - (1.0e-3f + fabsf(lo) * 1.0e-6f);
@@ -198,6 +253,17 @@ __device__ __forceinline__ float fit_hit_to_track(float tx, float ty, float x0, 
     return (float)cond * scatter + (float)(!cond) * FLT_MAX;
 }
 
+// One sensor's hits: coordinates in LDS (a ring slot) or in the event blob.
+struct SView {
+    const float* x;
+    const float* y;
+    const float* z;
+    int start;
+    int n;
+    bool sorted;
+    float zmin, zmax;
+};
+
 struct EventView {
     int S, N;
     const float* X;
@@ -205,107 +271,138 @@ struct EventView {
     const float* Zh;
 };
 
-// ---------------------------------------------------------------------------
-// (a) candidate windows of one hit h on sensor f >= 4 (Kernel.cl:73-172)
-// ---------------------------------------------------------------------------
-template <int B>
-__device__ int4 fill_one(const Shared<B>& sh, const EventView& ev, int h, int f) {
-    const int s2 = f - 2;
-    const bool p2 = f <= ev.S - 3;
-    const int st2 = sh.s_start[s2], n2 = sh.s_num[s2], end2 = st2 + n2;
-    const float x0 = ev.X[h], z0 = ev.Zh[h];
-    const float* X = ev.X;
-    const float* Zh = ev.Zh;
-    const bool sorted2 = sh.s_sorted[s2] != 0;
-    int4 c = make_int4(-1, -1, -1, -1);
-    if (n2 == 0) return c;
-
-    // h1 window: first index with |x1-x0| < 0.7|z1-z0|, then the first later
-    // index where that fails, else the sensor end (:115-135, :161-165).
-    if (sorted2 && sh.s_zmin[s2] == sh.s_zmax[s2]) {
-        // one z on the sensor: the predicate is -D < fl(x1-x0) < D
-        const float D = CLF_PARAM_MAXXSLOPE_CANDIDATES * fabsf(sh.s_zmin[s2] - z0);
-        const int lo = first_diff_gt(X, st2, end2, x0, -D);
-        const int hi = first_diff_ge(X, lo, end2, x0, D);
-        if (lo < hi) {
-            c.x = lo;
-            c.y = hi;
-        }
+template <int B, int C>
+__device__ __forceinline__ SView sensor_view(Shared<B, C>& sh, const EventView& ev, int s,
+                                             bool allow_lds) {
+    SView v;
+    v.start = sh.s_start[s];
+    v.n = sh.s_num[s];
+    v.sorted = sh.s_sorted[s] != 0;
+    v.zmin = sh.s_zmin[s];
+    v.zmax = sh.s_zmax[s];
+    if (allow_lds && v.n <= C) {
+        const int k = s % kRing;
+        v.x = sh.rx[k];
+        v.y = sh.ry[k];
+        v.z = sh.rz[k];
     } else {
-        int i = st2;
-        if (sorted2) {
-            // no hit with fl(x1-x0) <= -B can pass, B >= every 0.7|z1-z0|
-            const float a = fabsf(sh.s_zmin[s2] - z0), b = fabsf(sh.s_zmax[s2] - z0);
-            const float Bd = CLF_PARAM_MAXXSLOPE_CANDIDATES * fmaxf(a, b);
-            i = first_diff_gt(X, st2, end2, x0, -Bd);
-        }
-        for (; i < end2; ++i) {
-            const float dxmax = CLF_PARAM_MAXXSLOPE_CANDIDATES * fabsf(Zh[i] - z0);
-            if (fabsf(X[i] - x0) < dxmax) break;
-        }
-        if (i < end2) {
-            c.x = i;
-            for (++i; i < end2; ++i) {
-                const float dxmax = CLF_PARAM_MAXXSLOPE_CANDIDATES * fabsf(Zh[i] - z0);
-                if (!(fabsf(X[i] - x0) < dxmax)) break;
-            }
-            c.y = i;
-        }
+        v.x = ev.X + v.start;
+        v.y = ev.Y + v.start;
+        v.z = ev.Zh + v.start;
     }
-
-    // h2 window on the same sensor, for h acting as h1 (:85-102, :137-150)
-    if (p2) {
-        const int zs0 = sh.s_Z[f + 2];
-        const int zs2 = sh.s_Z[s2];
-        const float h_dist = fabsf(z0 - (float)zs0);
-        const float dxmax = CLF_PARAM_MAXXSLOPE_CANDIDATES * h_dist;
-        const float x0_min = x0 - dxmax;
-        const float x0_max = x0 + dxmax;
-        const float z2_tz = ((float)zs2 - (float)zs0) / (z0 - (float)zs0);
-        float xx = x0_max + (x0 - x0_max) * z2_tz;
-        const float xmin_h2 = xx - CLF_PARAM_TOLERANCE_CANDIDATES;
-        xx = x0_min + (x0 - x0_min) * z2_tz;
-        const float xmax_h2 = xx + CLF_PARAM_TOLERANCE_CANDIDATES;
-        if (sorted2) {
-            const int first = ub_gt(X, st2, end2, xmin_h2);
-            if (first < end2) {
-                c.z = first;
-                c.w = ub_gt(X, first + 1, end2, xmax_h2);
-            }
-        } else {
-            int i = st2;
-            for (; i < end2; ++i)
-                if (X[i] > xmin_h2) break;
-            if (i < end2) {
-                c.z = i;
-                for (++i; i < end2; ++i)
-                    if (X[i] > xmax_h2) break;
-                c.w = i;
-            }
-        }
-    }
-    return c;
+    return v;
 }
+
+// ---------------------------------------------------------------------------
+// (a) candidate windows (Kernel.cl:73-172), relative to s2.start, -1 = none
+// ---------------------------------------------------------------------------
+// h1 window of a hit (x0, z0) on sensor f into sensor f-2: the first index
+// with |x1-x0| < 0.7|z1-z0|, then the first later index where that fails,
+// else the sensor end (:115-135, :161-165).
+__device__ int2 window_h1(const SView& s2, float x0, float z0) {
+    int2 w = make_int2(-1, -1);
+    const int n = s2.n;
+    if (n == 0) return w;
+    const float* X = s2.x;
+    if (s2.sorted && s2.zmin == s2.zmax) {
+        // one z on the sensor: the predicate is -D < fl(x1-x0) < D
+        const float D = CLF_PARAM_MAXXSLOPE_CANDIDATES * fabsf(s2.zmin - z0);
+        const int lo = first_diff_gt(X, 0, n, x0, -D);
+        const int hi = first_diff_ge(X, lo, n, x0, D);
+        if (lo < hi) w = make_int2(lo, hi);
+        return w;
+    }
+    int i = 0;
+    if (s2.sorted) {
+        // no hit with fl(x1-x0) <= -Bd can pass, Bd >= every 0.7|z1-z0|
+        const float a = fabsf(s2.zmin - z0), b = fabsf(s2.zmax - z0);
+        const float Bd = CLF_PARAM_MAXXSLOPE_CANDIDATES * fmaxf(a, b);
+        i = first_diff_gt(X, 0, n, x0, -Bd);
+    }
+    const float* Z = s2.z;
+    for (; i < n; ++i) {
+        const float dxmax = CLF_PARAM_MAXXSLOPE_CANDIDATES * fabsf(Z[i] - z0);
+        if (fabsf(X[i] - x0) < dxmax) break;
+    }
+    if (i < n) {
+        w.x = i;
+        for (++i; i < n; ++i) {
+            const float dxmax = CLF_PARAM_MAXXSLOPE_CANDIDATES * fabsf(Z[i] - z0);
+            if (!(fabsf(X[i] - x0) < dxmax)) break;
+        }
+        w.y = i;
+    }
+    return w;
+}
+
+// h2 window of a hit (x0, z0) of sensor g acting as h1, into sensor g-2,
+// from the sensor Z integers zs0 = Z[g+2], zs2 = Z[g-2] (:85-102, :137-150).
+__device__ int2 window_h2(const SView& s2, float x0, float z0, int zs0, int zs2) {
+    int2 w = make_int2(-1, -1);
+    const float h_dist = fabsf(z0 - (float)zs0);
+    const float dxmax = CLF_PARAM_MAXXSLOPE_CANDIDATES * h_dist;
+    const float x0_min = x0 - dxmax;
+    const float x0_max = x0 + dxmax;
+    const float z2_tz = ((float)zs2 - (float)zs0) / (z0 - (float)zs0);
+    float xx = x0_max + (x0 - x0_max) * z2_tz;
+    const float xmin_h2 = xx - CLF_PARAM_TOLERANCE_CANDIDATES;
+    xx = x0_min + (x0 - x0_min) * z2_tz;
+    const float xmax_h2 = xx + CLF_PARAM_TOLERANCE_CANDIDATES;
+    const int n = s2.n;
+    const float* X = s2.x;
+    if (n == 0) return w;
+    if (s2.sorted) {
+        const int first = ub_gt(X, 0, n, xmin_h2);
+        if (first < n) w = make_int2(first, ub_gt(X, first + 1, n, xmax_h2));
+        return w;
+    }
+    int i = 0;
+    for (; i < n; ++i)
+        if (X[i] > xmin_h2) break;
+    if (i < n) {
+        w.x = i;
+        for (++i; i < n; ++i)
+            if (X[i] > xmax_h2) break;
+        w.y = i;
+    }
+    return w;
+}
+
+// ---------------------------------------------------------------------------
+// Used flags: LDS bitmask, or bytes in HBM for events above kUsedBits hits
+// ---------------------------------------------------------------------------
+struct UsedFlags {
+    uint32_t* bits;  // LDS
+    uint8_t* bytes;  // HBM
+    bool lds;
+    __device__ __forceinline__ bool get(int h) const {
+        return lds ? ((bits[h >> 5] >> (h & 31)) & 1u) != 0 : bytes[h] != 0;
+    }
+    __device__ __forceinline__ void set(int h) const {
+        if (lds) atomicOr(&bits[h >> 5], 1u << (h & 31));
+        else bytes[h] = 1;
+    }
+};
 
 // ---------------------------------------------------------------------------
 // The h0 gather of one sensor step, run by wave 0 (Kernel.cl:720-754).
 // Chunk c reads elements prev+lane; a lane whose element is used takes
 // values from the shared counter (which restarts at 0 each step and keeps
 // counting across chunks) and looks at prev + shift + value until it finds
-// an unused or out-of-range element.  The r-th such lane in lane order
-// takes the r-th of those elements.  Then prev = counter + shift and
-// shift += 64.  Visits are appended to v_h0 in lane order.
+// an unused or out-of-range element.  The r-th such lane in lane order takes
+// the r-th of those elements.  Then prev = counter + shift and shift += 64.
+// Visits (relative to sensor f) are appended to v_h0 in lane order.
 // ---------------------------------------------------------------------------
-template <int B>
-__device__ void gather_batch(Shared<B>& sh, const uint8_t* used, int start0, uint32_t hn0) {
+template <int B, int C>
+__device__ void gather_batch(Shared<B, C>& sh, const UsedFlags& used, int start0, uint32_t hn0) {
     const int lane = lane_id();
     uint32_t prev = sh.g_prev, shift = sh.g_shift, counter = sh.g_counter;
     int nv = 0;
     while (prev < hn0 && nv + kWave <= kMaxVisits) {
         const uint32_t e = prev + (uint32_t)lane;
         const bool inb = e < hn0;
-        const bool u = inb && used[start0 + (int)e];
-        int got = (inb && !u) ? start0 + (int)e : -1;
+        const bool u = inb && used.get(start0 + (int)e);
+        int got = (inb && !u) ? (int)e : -1;
         const uint64_t need = ballot(u);
         const int n_need = __popcll(need);
         const int rank = mbcnt(need);
@@ -314,10 +411,10 @@ __device__ void gather_batch(Shared<B>& sh, const uint8_t* used, int start0, uin
         while (done < n_need) {
             const uint32_t e2 = prev + shift + pos + (uint32_t)lane;
             const bool inb2 = e2 < hn0;
-            const bool un2 = inb2 && !used[start0 + (int)e2];
+            const bool un2 = inb2 && !used.get(start0 + (int)e2);
             const uint64_t sm = ballot(!inb2 || un2);
             const int c = __popcll(sm);
-            const int val = un2 ? start0 + (int)e2 : -1;
+            const int val = un2 ? (int)e2 : -1;
             const int k = rank - done;
             const bool mine = u && k >= 0 && k < c;
             const int src = kth_set_bit(sm, mine ? k : 0);
@@ -348,10 +445,10 @@ __device__ void gather_batch(Shared<B>& sh, const uint8_t* used, int start0, uin
 
 struct StepCtx {
     int f;
-    int st2, n2;  // sensor f-2
-    int st4, n4;  // sensor f-4
-    bool fast4;   // sensor f-4 x-sorted and the event in the supported domain
-    float zmin4, zmax4;
+    SView v0, v2, v4;  // sensors f, f-2, f-4
+    const int2* w2;    // h2 windows of sensor f-2 (relative to f-4)
+    bool fast4;        // sensor f-4 x-sorted and the event in the supported domain
+    bool seeds;        // sensors f-2 >= 4 have h2 windows
 };
 
 // ---------------------------------------------------------------------------
@@ -361,25 +458,25 @@ struct StepCtx {
 // skipped.  The winner minimises (fit, y, j, h2) lexicographically, which is
 // what the rows' strict-less first minima and the lowest-y tie-break pick.
 // ---------------------------------------------------------------------------
-template <int B>
-__device__ void seed_batch(Shared<B>& sh, const EventView& ev, const StepCtx& sc, const int4* cand,
-                           const uint8_t* used, uint4* tracklets, TtfEntry* nxt, int nv, int& n_next,
-                           int& n_tracklets) {
+template <int B, int C>
+__device__ void seed_batch(Shared<B, C>& sh, const StepCtx& sc, const UsedFlags& used,
+                           uint4* tracklets, TtfEntry* nxt, int nv, int& n_next, int& n_tracklets,
+                           PhaseClock& pc) {
     const int tid = threadIdx.x;
-    const float* X = ev.X;
-    const float* Y = ev.Y;
-    const float* Zh = ev.Zh;
+    const SView& v0 = sc.v0;
+    const SView& v2 = sc.v2;
+    const SView& v4 = sc.v4;
     // z of the first hit of sensor f-2 (:420-422)
-    const float s1_z = sc.n2 > 0 ? Zh[sc.st2] : 0.f;
+    const float s1_z = v2.n > 0 ? v2.z[0] : 0.f;
 
     for (int v = tid; v < nv; v += B) {
-        const int h0 = sh.v_h0[v];
-        const int4 c = cand[h0];
-        sh.v_first[v] = c.x;
-        sh.v_nh[v] = (int)(uint32_t)(c.y - c.x);
-        const float z0 = Zh[h0];
-        sh.v_x0[v] = X[h0];
-        sh.v_y0[v] = Y[h0];
+        const int e0 = sh.v_h0[v];
+        const float x0 = v0.x[e0], z0 = v0.z[e0];
+        const int2 w = window_h1(v2, x0, z0);
+        sh.v_first[v] = w.x;
+        sh.v_nh[v] = (int)(uint32_t)(w.y - w.x);
+        sh.v_x0[v] = x0;
+        sh.v_y0[v] = v0.y[e0];
         sh.v_z0[v] = z0;
         sh.v_dymax[v] = CLF_PARAM_MAXYSLOPE * fabsf(s1_z - z0);
 #pragma unroll
@@ -401,10 +498,11 @@ __device__ void seed_batch(Shared<B>& sh, const EventView& ev, const StepCtx& sc
     }
     __syncthreads();
     const int P = sh.v_pre[nv];
+    prof_mark(sh, pc, PROF_SEED_SETUP);
+    prof_add(sh, pc, CNT_PAIRS, (unsigned long long)P);
 
-    // pairs (visit v, window element e) in a flat index
-    auto visit_of = [&](int p) {
-        int lo = 0, hi = nv;  // last v with v_pre[v] <= p
+    auto visit_of = [&](int p) {  // last v with v_pre[v] <= p
+        int lo = 0, hi = nv;
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
             if (sh.v_pre[mid] <= p) lo = mid;
@@ -418,105 +516,129 @@ __device__ void seed_batch(Shared<B>& sh, const EventView& ev, const StepCtx& sc
         const int v = visit_of(p);
         const int e = p - sh.v_pre[v];
         const int h1 = sh.v_first[v] + e;
-        if (cand[h1].z == -1)
+        if (!sc.seeds || sc.w2[h1].x == -1)
             atomicMin(&sh.v_death[v * CLF_SEED_ROWS + (e & (CLF_SEED_ROWS - 1))], e / CLF_SEED_ROWS);
     }
     __syncthreads();
+    prof_mark(sh, pc, PROF_SEED_STOP);
 
-    const float* X4 = X + sc.st4;
-    const float* Y4 = Y + sc.st4;
-    const float* Z4 = Zh + sc.st4;
-    for (int p = tid; p < P; p += B) {
-        const int v = visit_of(p);
-        const int e = p - sh.v_pre[v];
-        const int y = e & (CLF_SEED_ROWS - 1);
-        const int j = e / CLF_SEED_ROWS;
-        if (j >= sh.v_death[v * CLF_SEED_ROWS + y]) continue;
-        const int h1 = sh.v_first[v] + e;
-        if (used[h1]) continue;
-        const float x0 = sh.v_x0[v], y0 = sh.v_y0[v], z0 = sh.v_z0[v];
-        const float x1 = X[h1], y1 = Y[h1], z1 = Zh[h1];
-        if (!(fabsf(y1 - y0) < sh.v_dymax[v])) continue;  // :509, independent of h2
-        const int4 c = cand[h1];
-        const float dz_inverted = 1.f / (z1 - z0);
-        int k0 = c.z - sc.st4, k1 = c.w - sc.st4;
-        if (sc.fast4) {
-            const float pa = x0 + (x1 - x0) * ((sc.zmin4 - z0) * dz_inverted);
-            const float pb = x0 + (x1 - x0) * ((sc.zmax4 - z0) * dz_inverted);
-            float lo, hi;
-            x_window(pa, pb, lo, hi);
-            k0 = lb_ge(X4, k0, k1, lo);
-            k1 = ub_gt(X4, k0, k1, hi);
-        }
-        float best = FLT_MAX;
-        int bk = -1;
-        for (int k = k0; k < k1; ++k) {
-            const float x2 = X4[k], y2 = Y4[k], z2 = Z4[k];
-            const float z2_tz = (z2 - z0) * dz_inverted;
-            const float x = x0 + (x1 - x0) * z2_tz;
-            const float yy = y0 + (y1 - y0) * z2_tz;
-            const float dx = x - x2;
-            const float dy = yy - y2;
-            if (fabsf(dx) < CLF_PARAM_TOLERANCE && fabsf(dy) < CLF_PARAM_TOLERANCE) {
-                const float scatterNum = (dx * dx) + (dy * dy);
-                const float scatterDenom = 1.f / (z2 - z1);
-                const float scatter = scatterNum * scatterDenom * scatterDenom;
-                if (scatter < CLF_MAX_SCATTER && scatter < best) {
-                    best = scatter;
-                    bk = k;
+    if (sc.seeds) {
+        for (int p = tid; p < P; p += B) {
+            const int v = visit_of(p);
+            const int e = p - sh.v_pre[v];
+            const int y = e & (CLF_SEED_ROWS - 1);
+            const int j = e / CLF_SEED_ROWS;
+            if (j >= sh.v_death[v * CLF_SEED_ROWS + y]) continue;
+            const int h1 = sh.v_first[v] + e;
+            if (used.get(v2.start + h1)) continue;
+            const float x0 = sh.v_x0[v], y0 = sh.v_y0[v], z0 = sh.v_z0[v];
+            const float x1 = v2.x[h1], y1 = v2.y[h1], z1 = v2.z[h1];
+            if (!(fabsf(y1 - y0) < sh.v_dymax[v])) continue;  // :509, independent of h2
+            const int2 c = sc.w2[h1];
+            const float dz_inverted = 1.f / (z1 - z0);
+            int k0 = c.x, k1 = c.y;
+            if (sc.fast4) {
+                const float pa = x0 + (x1 - x0) * ((v4.zmin - z0) * dz_inverted);
+                const float pb = x0 + (x1 - x0) * ((v4.zmax - z0) * dz_inverted);
+                float lo, hi;
+                x_window(pa, pb, lo, hi);
+                k0 = lb_ge(v4.x, k0, k1, lo);
+                k1 = ub_gt(v4.x, k0, k1, hi);
+            }
+            float best = FLT_MAX;
+            int bk = -1;
+            for (int k = k0; k < k1; ++k) {
+                const float x2 = v4.x[k], y2 = v4.y[k], z2 = v4.z[k];
+                const float z2_tz = (z2 - z0) * dz_inverted;
+                const float x = x0 + (x1 - x0) * z2_tz;
+                const float yy = y0 + (y1 - y0) * z2_tz;
+                const float dx = x - x2;
+                const float dy = yy - y2;
+                if (fabsf(dx) < CLF_PARAM_TOLERANCE && fabsf(dy) < CLF_PARAM_TOLERANCE) {
+                    const float scatterNum = (dx * dx) + (dy * dy);
+                    const float scatterDenom = 1.f / (z2 - z1);
+                    const float scatter = scatterNum * scatterDenom * scatterDenom;
+                    if (scatter < CLF_MAX_SCATTER && scatter < best) {
+                        best = scatter;
+                        bk = k;
+                    }
                 }
             }
-        }
-        if (bk >= 0) {
-            const unsigned long long key = ((unsigned long long)__float_as_uint(best + 0.0f) << 32) |
-                                           ((unsigned long long)y << 30) |
-                                           ((unsigned long long)j << 16) | (unsigned long long)bk;
-            atomicMin(&sh.v_best[v], key);
+            if (bk >= 0) {
+                const unsigned long long key =
+                    ((unsigned long long)__float_as_uint(best + 0.0f) << 32) |
+                    ((unsigned long long)y << 30) | ((unsigned long long)j << 16) |
+                    (unsigned long long)bk;
+                atomicMin(&sh.v_best[v], key);
+            }
         }
     }
     __syncthreads();
+    prof_mark(sh, pc, PROF_SEED_EVAL);
 
     // tracklets and their ttf entries, in visit order (:550-566)
     for (int base = 0; base < nv; base += B) {
         const int v = base + tid;
         const bool acc = v < nv && sh.v_best[v] != ~0ull;
         int pos[3], tot[3];
-        block_scan3<B>(sh, acc, false, false, pos, tot);
+        block_scan3<B, C>(sh, acc, false, false, pos, tot);
         if (acc) {
             const unsigned long long key = sh.v_best[v];
             const int y = (int)((key >> 30) & 3ull);
             const int j = (int)((key >> 16) & 0x3fffull);
             const int k = (int)(key & 0xffffull);
-            const int h0 = sh.v_h0[v];
             const int h1 = sh.v_first[v] + CLF_SEED_ROWS * j + y;
-            const int h2 = sc.st4 + k;
             const int idx = n_tracklets + pos[0];
-            tracklets[idx] = make_uint4((uint32_t)h0, (uint32_t)h1, (uint32_t)h2, 0u);
+            tracklets[idx] = make_uint4((uint32_t)(v0.start + sh.v_h0[v]), (uint32_t)(v2.start + h1),
+                                        (uint32_t)(v4.start + k), 0u);
             TtfEntry t;
             t.word = CLF_TTF_TRACKLET_BIT | (uint32_t)idx;
             t.nhits = 3;
-            t.x0 = X[h1];
-            t.y0 = Y[h1];
-            t.z0 = Zh[h1];
-            t.x1 = X[h2];
-            t.y1 = Y[h2];
-            t.z1 = Zh[h2];
+            t.x0 = v2.x[h1];
+            t.y0 = v2.y[h1];
+            t.z0 = v2.z[h1];
+            t.x1 = v4.x[k];
+            t.y1 = v4.y[k];
+            t.z1 = v4.z[k];
             nxt[n_next + pos[0]] = t;
         }
         n_tracklets += tot[0];
         n_next += tot[0];
+    }
+    prof_mark(sh, pc, PROF_SEED_EMIT);
+}
+
+// Copies sensor s into its ring slot when it fits (all threads).
+template <int B, int C>
+__device__ __forceinline__ void stage_sensor(Shared<B, C>& sh, const EventView& ev, int s) {
+    if (s < 0) return;
+    const int n = sh.s_num[s];
+    if (n > C) return;
+    const int b = sh.s_start[s], k = s % kRing;
+    for (int i = threadIdx.x; i < n; i += B) {
+        sh.rx[k][i] = ev.X[b + i];
+        sh.ry[k][i] = ev.Y[b + i];
+        sh.rz[k][i] = ev.Zh[b + i];
     }
 }
 
 // ---------------------------------------------------------------------------
 // One event
 // ---------------------------------------------------------------------------
-template <int B>
-__device__ void process_event(const SearchArgs& a, int e, Shared<B>& sh) {
-    constexpr int W = Shared<B>::W;
+template <int B, int C>
+__device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
+    constexpr int W = Shared<B, C>::W;
     const int tid = threadIdx.x, lane = lane_id(), wave = wave_id();
     const EventDesc d = a.ev[e];
     EventResult* res = a.results + e;
+    PhaseClock pc;
+    pc.on = a.prof != nullptr;
+    pc.t = 0;
+    if (pc.on && tid == 0) {
+        for (int k = 0; k < kProfSlots; ++k) sh.prof[k] = 0;
+        pc.t = wall_clock64();
+    }
+    const unsigned long long t_start = pc.t;
     if (d.n_hits < 0) {
         if (tid == 0) {
             EventResult r = {};
@@ -536,22 +658,25 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B>& sh) {
     ev.X = (const float*)(gNum + S + N);
     ev.Y = ev.X + N;
     ev.Zh = ev.Y + N;
-    int4* cand = a.cand + d.hit_off;
-    uint8_t* used = a.used + d.hit_off;
     clf_track* tracks = a.tracks + d.track_off;
     uint4* tracklets = a.tracklets + d.track_off;
     TtfEntry* cur = a.ttf_a + d.track_off;
     TtfEntry* nxt = a.ttf_b + d.track_off;
     uint32_t* weak = a.weak + d.track_off;
+    UsedFlags used;
+    used.lds = N <= kUsedBits;
+    used.bits = sh.used_bits;
+    used.bytes = a.used + d.hit_off;
 
     for (int s = tid; s < S; s += B) {
         sh.s_start[s] = gStart[s];
         sh.s_num[s] = gNum[s];
         sh.s_Z[s] = gZ[s];
     }
-    for (int i = tid; i < N; i += B) {
-        used[i] = 0;
-        cand[i] = make_int4(-1, -1, -1, -1);  // the host's -1 fill (KernelInvoker.cpp:162-163)
+    if (used.lds) {
+        for (int i = tid; i < (N + 31) / 32; i += B) sh.used_bits[i] = 0u;
+    } else {
+        for (int i = tid; i < N; i += B) used.bytes[i] = 0;
     }
     if (tid == 0) {
         sh.flags = 0;
@@ -601,34 +726,28 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B>& sh) {
         if (sh.s_num[f] > 0 && zs0 >= sh.s_zmin[f] && zs0 <= sh.s_zmax[f])
             atomicOr(&sh.flags, CLF_FLAG_DEGENERATE_Z);
     }
-    if (tid == 0) {
-        int acc = 0;
-        for (int f = 0; f < S; ++f) {
-            sh.s_pre[f] = acc;
-            if (f >= 4) acc += sh.s_num[f];
-        }
-        sh.s_pre[S] = acc;
-    }
     __syncthreads();
     const bool degenerate = (sh.flags & CLF_FLAG_DEGENERATE_Z) != 0;
 
-    // (a) candidate windows of every hit on sensors 4..S-1
-    {
-        const int total = sh.s_pre[S];
-        for (int p = tid; p < total; p += B) {
-            int lo = 4, hi = S;  // last f with s_pre[f] <= p
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (sh.s_pre[mid] <= p) lo = mid;
-                else hi = mid;
-            }
-            const int f = lo;
-            const int h = sh.s_start[f] + (p - sh.s_pre[f]);
-            cand[h] = fill_one<B>(sh, ev, h, f);
-        }
-    }
-    __syncthreads();
     if (a.fill_only) {
+        // stage (a) alone, from HBM: every hit's windows in the reference's
+        // absolute-index form (dev_hit_candidates / dev_hit_h2_candidates)
+        int4* cand = a.cand + d.hit_off;
+        for (int i = tid; i < N; i += B) cand[i] = make_int4(-1, -1, -1, -1);
+        __syncthreads();
+        for (int f = 4; f < S; ++f) {
+            const SView v2 = sensor_view<B, C>(sh, ev, f - 2, false);
+            const int st = sh.s_start[f], n = sh.s_num[f];
+            const bool p2 = f <= S - 3;
+            for (int i = tid; i < n; i += B) {
+                const float x0 = ev.X[st + i], z0 = ev.Zh[st + i];
+                const int2 w1 = window_h1(v2, x0, z0);
+                int2 w2 = make_int2(-1, -1);
+                if (p2) w2 = window_h2(v2, x0, z0, sh.s_Z[f + 2], sh.s_Z[f - 2]);
+                cand[st + i] = make_int4(w1.x < 0 ? -1 : v2.start + w1.x, w1.x < 0 ? -1 : v2.start + w1.y,
+                                         w2.x < 0 ? -1 : v2.start + w2.x, w2.x < 0 ? -1 : v2.start + w2.y);
+            }
+        }
         if (tid == 0) {
             EventResult r = {};
             r.flags = sh.flags;
@@ -637,20 +756,35 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B>& sh) {
         return;
     }
 
+    // the ring starts with sensors S-1 .. S-4
+    for (int s = S - 1; s >= S - 4 && s >= 0; --s) stage_sensor<B, C>(sh, ev, s);
+    prof_mark(sh, pc, PROF_PROLOGUE);
+
     int n_cur = 0, n_tracks = 0, n_tracklets = 0, n_weak = 0, visits = 0;
+    int2* w2_hbm = reinterpret_cast<int2*>(a.cand + d.hit_off);  // h2 windows of oversized sensors
     for (int f = S - 1; f >= 4; --f) {
+        stage_sensor<B, C>(sh, ev, f - 4);
+        __syncthreads();
+        prof_mark(sh, pc, PROF_STAGE);
+        prof_add(sh, pc, CNT_FWD_ENTRIES, (unsigned long long)n_cur);
+        prof_add(sh, pc, CNT_FWD_TILES, (unsigned long long)((n_cur + B - 1) / B));
         StepCtx sc;
         sc.f = f;
-        sc.st2 = sh.s_start[f - 2];
-        sc.n2 = sh.s_num[f - 2];
-        sc.st4 = sh.s_start[f - 4];
-        sc.n4 = sh.s_num[f - 4];
-        sc.fast4 = sh.s_sorted[f - 4] && !degenerate;
-        sc.zmin4 = sh.s_zmin[f - 4];
-        sc.zmax4 = sh.s_zmax[f - 4];
-        const float* X4 = ev.X + sc.st4;
-        const float* Y4 = ev.Y + sc.st4;
-        const float* Z4 = ev.Zh + sc.st4;
+        sc.v0 = sensor_view<B, C>(sh, ev, f, true);
+        sc.v2 = sensor_view<B, C>(sh, ev, f - 2, true);
+        sc.v4 = sensor_view<B, C>(sh, ev, f - 4, true);
+        sc.fast4 = sc.v4.sorted && !degenerate;
+        sc.seeds = f - 2 >= 4;
+        int2* w2 = sc.v2.n <= C ? sh.w2 : w2_hbm + sc.v2.start;
+        sc.w2 = w2;
+        const SView& v4 = sc.v4;
+
+        // (a) h2 windows of sensor f-2's hits into sensor f-4
+        if (sc.seeds) {
+            const int zs0 = sh.s_Z[f], zs2 = sh.s_Z[f - 4];
+            for (int i = tid; i < sc.v2.n; i += B)
+                w2[i] = window_h2(v4, sc.v2.x[i], sc.v2.z[i], zs0, zs2);
+        }
 
         // (c) forwarding of the ttf entries of step f+1 (Kernel.cl:210-367)
         int n_next = 0;
@@ -669,19 +803,19 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B>& sh) {
                 const float tyn = (t.y1 - t.y0);
                 const float tx = txn * td;
                 const float ty = tyn * td;
-                int k0 = 0, k1 = sc.n4;
-                if (sc.fast4 && sc.n4 > 0) {
-                    const float pa = t.x0 + tx * (sc.zmin4 - t.z0);
-                    const float pb = t.x0 + tx * (sc.zmax4 - t.z0);
+                int k0 = 0, k1 = v4.n;
+                if (sc.fast4 && v4.n > 0) {
+                    const float pa = t.x0 + tx * (v4.zmin - t.z0);
+                    const float pb = t.x0 + tx * (v4.zmax - t.z0);
                     float lo, hi;
                     x_window(pa, pb, lo, hi);
-                    k0 = lb_ge(X4, 0, sc.n4, lo);
-                    k1 = ub_gt(X4, k0, sc.n4, hi);
+                    k0 = lb_ge(v4.x, 0, v4.n, lo);
+                    k1 = ub_gt(v4.x, k0, v4.n, hi);
                 }
                 float best_fit = FLT_MAX;
                 for (int k = k0; k < k1; ++k) {
                     const float fit =
-                        fit_hit_to_track(tx, ty, t.x0, t.y0, t.z0, t.z1, X4[k], Y4[k], Z4[k]);
+                        fit_hit_to_track(tx, ty, t.x0, t.y0, t.z0, t.z1, v4.x[k], v4.y[k], v4.z[k]);
                     const bool better = fit < best_fit;
                     best_fit = (float)better * fit + (float)(!better) * best_fit;
                     bk = better ? k : bk;
@@ -703,18 +837,18 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B>& sh) {
                 }
             }
             int pos[3], tot[3];
-            block_scan3<B>(sh, f_new, f_req, f_weak, pos, tot);
+            block_scan3<B, C>(sh, f_new, f_req, f_weak, pos, tot);
             if (active) {
                 const uint32_t idx = t.word & CLF_TTF_INDEX_MASK;
                 if (found) {
-                    const int b = sc.st4 + bk;
-                    used[b] = 1;
+                    const int b = v4.start + bk;
+                    used.set(b);
                     uint32_t slot;
                     if (f_new) {
                         const uint4 tl = tracklets[idx];
-                        used[tl.x] = 1;
-                        used[tl.y] = 1;
-                        used[tl.z] = 1;
+                        used.set((int)tl.x);
+                        used.set((int)tl.y);
+                        used.set((int)tl.z);
                         slot = (uint32_t)(n_tracks + pos[0]);
                         uint32_t* r = reinterpret_cast<uint32_t*>(tracks + slot);
                         r[0] = 4;
@@ -736,9 +870,9 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B>& sh) {
                         n.x0 = t.x1;
                         n.y0 = t.y1;
                         n.z0 = t.z1;
-                        n.x1 = X4[bk];
-                        n.y1 = Y4[bk];
-                        n.z1 = Z4[bk];
+                        n.x1 = v4.x[bk];
+                        n.y1 = v4.y[bk];
+                        n.z1 = v4.z[bk];
                         nxt[n_next + pos[1]] = n;
                     } else {
                         atomicOr(&sh.flags, CLF_FLAG_TRACK_FULL);
@@ -757,26 +891,30 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B>& sh) {
             n_next += tot[1];
             n_weak += tot[2];
         }
-        __syncthreads();  // used[] of this step visible to the gather and seeding
-
-        // (b) gather + seeding (Kernel.cl:720-764)
         if (tid == 0) {
             sh.g_prev = 0;
             sh.g_shift = CLF_GATHER_CHUNK;
             sh.g_counter = 0;
         }
-        __syncthreads();
-        const int start0 = sh.s_start[f];
-        const uint32_t hn0 = (uint32_t)sh.s_num[f];
+        __syncthreads();  // used flags and h2 windows of this step visible to seeding
+        prof_mark(sh, pc, PROF_FORWARD);
+
+        // (b) gather + seeding (Kernel.cl:720-764)
+        const int start0 = sc.v0.start;
+        const uint32_t hn0 = (uint32_t)sc.v0.n;
         for (;;) {
-            if (wave == 0) gather_batch<B>(sh, used, start0, hn0);
+            if (wave == 0) gather_batch<B, C>(sh, used, start0, hn0);
             __syncthreads();
+            prof_mark(sh, pc, PROF_GATHER);
             const int nv = sh.g_nv;
             const bool more = sh.g_more != 0;
             visits += nv;
+            prof_add(sh, pc, CNT_VISITS, (unsigned long long)nv);
+            prof_add(sh, pc, CNT_BATCHES, 1ull);
             if (nv > 0)
-                seed_batch<B>(sh, ev, sc, cand, used, tracklets, nxt, nv, n_next, n_tracklets);
+                seed_batch<B, C>(sh, sc, used, tracklets, nxt, nv, n_next, n_tracklets, pc);
             __syncthreads();
+            prof_mark(sh, pc, PROF_SEED_EMIT);
             if (!more) break;
         }
 
@@ -789,7 +927,6 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B>& sh) {
         cur = nxt;
         nxt = tmp;
         n_cur = n_next;
-        __syncthreads();
     }
 
     // tail: tracklet entries of the last step become weak (Kernel.cl:771-792)
@@ -798,7 +935,7 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B>& sh) {
         uint32_t w = 0;
         const bool tl = i < n_cur && ((w = cur[i].word) & CLF_TTF_TRACKLET_BIT) != 0;
         int pos[3], tot[3];
-        block_scan3<B>(sh, tl, false, false, pos, tot);
+        block_scan3<B, C>(sh, tl, false, false, pos, tot);
         if (tl) weak[n_weak + pos[0]] = w & CLF_TTF_INDEX_MASK;
         n_weak += tot[0];
     }
@@ -811,10 +948,10 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B>& sh) {
         bool ok = false;
         if (i < n_weak) {
             tl = tracklets[weak[i]];
-            ok = !used[tl.x] && !used[tl.y] && !used[tl.z];
+            ok = !used.get((int)tl.x) && !used.get((int)tl.y) && !used.get((int)tl.z);
         }
         int pos[3], tot[3];
-        block_scan3<B>(sh, ok, false, false, pos, tot);
+        block_scan3<B, C>(sh, ok, false, false, pos, tot);
         if (ok) {
             uint32_t* r = reinterpret_cast<uint32_t*>(tracks + n_tracks + pos[0]);
             r[0] = 3;
@@ -844,11 +981,16 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B>& sh) {
         r.reserved = 0;
         *res = r;
     }
+    prof_mark(sh, pc, PROF_TAIL);
+    if (pc.on && tid == 0) {
+        sh.prof[PROF_TOTAL] = pc.t - t_start;
+        for (int k = 0; k < kProfSlots; ++k) a.prof[(size_t)e * kProfSlots + k] = sh.prof[k];
+    }
 }
 
-template <int B>
+template <int B, int C>
 __global__ __launch_bounds__(B) void search_kernel(SearchArgs a) {
-    __shared__ Shared<B> sh;
+    __shared__ Shared<B, C> sh;
     for (;;) {
         if (threadIdx.x == 0) sh.item = atomicAdd(a.queue_head, 1);
         __syncthreads();
@@ -857,7 +999,7 @@ __global__ __launch_bounds__(B) void search_kernel(SearchArgs a) {
         if (item >= a.n_items) break;  // every wave of the group leaves together
         const int idx = a.first_item + item;
         const int e = a.order ? a.order[idx] : idx;
-        process_event<B>(a, e, sh);
+        process_event<B, C>(a, e, sh);
         __syncthreads();
     }
 }
@@ -923,30 +1065,43 @@ __global__ __launch_bounds__(256) void probe_copy_kernel(const uint4* __restrict
 // ---------------------------------------------------------------------------
 // Launchers (called from the host library)
 // ---------------------------------------------------------------------------
-hipError_t launch_probe_copy(const void* src, void* dst, int64_t bytes, hipStream_t stream) {
-    const int64_t n = bytes / 16;
-    if (n <= 0) return hipSuccess;
-    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 256 * 32);
-    hipLaunchKernelGGL(probe_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, stream,
-                       (const uint4*)src, (uint4*)dst, n);
-    return hipGetLastError();
+#define CLF_FOR_EACH_CONFIG(X) \
+    X(64, 256)                 \
+    X(128, 256)                \
+    X(256, 256)                \
+    X(512, 256)                \
+    X(64, 512)                 \
+    X(128, 512)                \
+    X(256, 512)                \
+    X(512, 512)
+
+bool config_supported(int block, int lds_hits) {
+#define CLF_CASE(B_, C_) \
+    if (block == B_ && lds_hits == C_) return true;
+    CLF_FOR_EACH_CONFIG(CLF_CASE)
+#undef CLF_CASE
+    return false;
 }
 
-hipError_t launch_search(const SearchArgs& a, int block, int grid, hipStream_t stream) {
-    switch (block) {
-        case 64:
-            hipLaunchKernelGGL(search_kernel<64>, dim3(grid), dim3(64), 0, stream, a);
-            break;
-        case 128:
-            hipLaunchKernelGGL(search_kernel<128>, dim3(grid), dim3(128), 0, stream, a);
-            break;
-        case 256:
-            hipLaunchKernelGGL(search_kernel<256>, dim3(grid), dim3(256), 0, stream, a);
-            break;
-        default:
-            return hipErrorInvalidValue;
+hipError_t launch_search(const SearchArgs& a, int block, int lds_hits, int grid, hipStream_t stream) {
+#define CLF_CASE(B_, C_)                                                                   \
+    if (block == B_ && lds_hits == C_) {                                                   \
+        hipLaunchKernelGGL((search_kernel<B_, C_>), dim3(grid), dim3(B_), 0, stream, a); \
+        return hipGetLastError();                                                          \
     }
-    return hipGetLastError();
+    CLF_FOR_EACH_CONFIG(CLF_CASE)
+#undef CLF_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t occupancy(int block, int lds_hits, int* blocks_per_cu) {
+#define CLF_CASE(B_, C_)                                                                    \
+    if (block == B_ && lds_hits == C_)                                                      \
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,                  \
+                                                            search_kernel<B_, C_>, B_, 0);
+    CLF_FOR_EACH_CONFIG(CLF_CASE)
+#undef CLF_CASE
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_zero_i32(int32_t* p, int n, hipStream_t stream) {
@@ -964,17 +1119,13 @@ hipError_t launch_pack(const EventDesc* ev, const EventResult* res, int n, int64
     return hipGetLastError();
 }
 
-hipError_t occupancy(int block, int* blocks_per_cu) {
-    switch (block) {
-        case 64:
-            return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, search_kernel<64>, 64, 0);
-        case 128:
-            return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, search_kernel<128>, 128, 0);
-        case 256:
-            return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, search_kernel<256>, 256, 0);
-        default:
-            return hipErrorInvalidValue;
-    }
+hipError_t launch_probe_copy(const void* src, void* dst, int64_t bytes, hipStream_t stream) {
+    const int64_t n = bytes / 16;
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = (n + 255) / 256 < 256 * 32 ? (n + 255) / 256 : 256 * 32;
+    hipLaunchKernelGGL(probe_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, stream,
+                       (const uint4*)src, (uint4*)dst, n);
+    return hipGetLastError();
 }
 
 }  // namespace clf

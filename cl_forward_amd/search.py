@@ -192,6 +192,26 @@ class Batch:
             stats.append({k: int(getattr(s, k)) for k, _ in EventStats._fields_ if k != "reserved"})
         return outs, stats
 
+    PROFILE_FIELDS = ("prologue", "stage", "forward", "gather", "seed_setup", "seed_stop",
+                      "seed_eval", "seed_emit", "tail", "total", "fwd_entries", "visits", "pairs",
+                      "batches", "fwd_tiles", "unused")
+
+    def profile(self, mode: int = MODE_BATCHED):
+        """Per-event phase times (ns) and work counters of one run, as a dict of
+        numpy arrays keyed by PROFILE_FIELDS."""
+        import numpy as np
+
+        out = np.zeros((max(self.n, 1), 16), np.uint64)
+        tick = ctypes.c_double()
+        _lib.check(self._L.clf_batch_profile(self.ctx.handle, self._h, int(mode),
+                                             out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                             ctypes.byref(tick)), "clf_batch_profile")
+        out = out[: self.n].astype(np.float64)
+        res = {}
+        for k, name in enumerate(self.PROFILE_FIELDS):
+            res[name] = out[:, k] * (tick.value if k < 10 else 1.0)
+        return res
+
     def time(self, mode: int = MODE_BATCHED, warmup: int = 2, reps: int = 5) -> dict:
         t = Timing()
         _lib.check(self._L.clf_batch_time(self.ctx.handle, self._h, int(mode), int(warmup),

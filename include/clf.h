@@ -60,10 +60,12 @@ typedef struct clf_event_stats {
 
 /* Launch parameters (no effect on results). */
 typedef struct clf_launch_config {
-    int32_t block_size; /* 64, 128 or 256 threads per event workgroup      */
-    int32_t grid_size;  /* persistent workgroups; 0 = CUs x occupancy      */
-    int32_t grain;      /* events per dequeue (1)                          */
-    int32_t use_graph;  /* batched mode: capture and replay a hipGraph     */
+    int32_t block_size; /* 64, 128, 256 or 512 threads per event workgroup   */
+    int32_t grid_size;  /* persistent workgroups; 0 = CUs x occupancy        */
+    int32_t grain;      /* events per dequeue (1)                            */
+    int32_t use_graph;  /* batched mode: capture and replay a hipGraph       */
+    int32_t lds_hits;   /* hits per LDS sensor slot, 256 or 512; a sensor
+                           with more hits is read from HBM                   */
 } clf_launch_config;
 
 /* Results of a batch; memory owned by the batch, valid until the batch
@@ -139,6 +141,15 @@ clf_status clf_search(clf_context* ctx, clf_mode mode, int32_t n_events,
 clf_status clf_fill_candidates(clf_context* ctx, int32_t n_events, const uint8_t* const* events,
                                const size_t* sizes, int32_t* const* cand_h1,
                                int32_t* const* cand_h2, uint32_t* flags);
+
+/* Measurement probe, no counterpart in the reference: runs the batch once
+ * with per-event phase timers and work counters on.  out receives 16
+ * uint64 per event: wall-clock ticks of prologue, staging, forwarding,
+ * gather, seeding set-up, row stops, pair evaluation, emission, tail, total,
+ * then forwarded entries, gathered h0, (h0,h1) pairs, seeding batches and
+ * forwarding tiles.  tick_ns receives the length of one tick. */
+clf_status clf_batch_profile(clf_context* ctx, clf_batch* batch, clf_mode mode, uint64_t* out,
+                             double* tick_ns);
 
 /* Measurement probe, no counterpart in the reference: copies `bytes`
  * (a multiple of 16) between two scratch buffers with a plain 16-byte-per-lane

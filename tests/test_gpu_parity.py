@@ -85,14 +85,24 @@ def test_search_plain51_both_modes(gpu_ctx, oracle_mod, plain51_blobs):
     assert a == b
 
 
-@pytest.mark.parametrize("block", [64, 128, 256])
-def test_block_sizes_identical(oracle_mod, plain51_blobs, block):
+@pytest.fixture(scope="module")
+def config_blobs(plain51_blobs, mcdata_sorted):
+    # plain events, two mcdata events and two stress events whose largest
+    # sensors (over 256 hits) exceed a 256-hit LDS slot and are read from HBM
+    blobs = plain51_blobs[:6] + [mcdata_sorted["77"], mcdata_sorted["3"]]
+    blobs += [generator.stress_event(i).pack() for i in (0, 7)]
+    return blobs
+
+
+@pytest.mark.parametrize("block", [64, 128, 256, 512])
+@pytest.mark.parametrize("lds_hits", [256, 512])
+def test_launch_configs_identical(oracle_mod, config_blobs, block, lds_hits):
     from cl_forward_amd.search import Context
 
-    _say(f"search: block size {block}")
+    _say(f"search: block size {block}, lds slot {lds_hits} hits")
     with Context(0) as ctx:
-        ctx.set_config(block_size=block)
-        _check_search(ctx, oracle_mod, plain51_blobs[:12], f"block{block}")
+        ctx.set_config(block_size=block, lds_hits=lds_hits)
+        _check_search(ctx, oracle_mod, config_blobs, f"block{block}/lds{lds_hits}")
 
 
 def test_determinism_and_graph_replays(gpu_ctx, plain51_blobs):
