@@ -101,6 +101,7 @@ def main(argv=None):
                     choices=["batch1000", "stress1000", "plain51", "single"])
     ap.add_argument("--block", type=int, default=0, help="override the tuned workgroup size")
     ap.add_argument("--grid", type=int, default=-1, help="override the tuned persistent grid")
+    ap.add_argument("--lds-hits", type=int, default=0, help="override the tuned LDS slot size")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -145,6 +146,8 @@ def main(argv=None):
         cfg["block_size"] = a.block
     if a.grid >= 0:
         cfg["grid_size"] = a.grid
+    if a.lds_hits:
+        cfg["lds_hits"] = a.lds_hits
     cfg["use_graph"] = 0 if a.no_graph else 1
     ctx.set_config(**cfg)
     cfg = ctx.config()
@@ -248,6 +251,7 @@ def main(argv=None):
             "path": "batched persistent kernel, hipGraph replay, inputs resident in HBM",
             "block_size": cfg["block_size"],
             "grid_size": cfg["grid_size"],
+            "lds_hits": cfg["lds_hits"],
             "flags_seen": flags_any,
         },
         "device_ms_per_step": round(dev["ms_run"], 4),

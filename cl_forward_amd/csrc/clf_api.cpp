@@ -148,7 +148,7 @@ struct clf_batch {
     int32_t n = 0;
     int64_t total_hits = 0, total_cap = 0;
     size_t blob_bytes = 0;
-    DevBuf blobs, desc, order, queue, cand, used, tracks, tracklets, ttf_a, ttf_b, weak, results,
+    DevBuf blobs, desc, order, queue, cand, used, tracks, ttf_a, ttf_b, weak, results,
         prefix, packed;
     PinnedBuf h_blobs, h_results, h_prefix, h_packed;
     std::vector<EventDesc> h_desc;
@@ -186,10 +186,9 @@ SearchArgs make_args(clf_batch* b) {
     a.cand = b->cand.as<int4>();
     a.used = b->used.as<uint8_t>();
     a.tracks = b->tracks.as<clf_track>();
-    a.tracklets = b->tracklets.as<uint4>();
     a.ttf_a = b->ttf_a.as<TtfEntry>();
     a.ttf_b = b->ttf_b.as<TtfEntry>();
-    a.weak = b->weak.as<uint32_t>();
+    a.weak = b->weak.as<uint4>();
     a.results = b->results.as<EventResult>();
     a.fill_only = 0;
     a.prof = nullptr;
@@ -360,7 +359,7 @@ void clf_batch_destroy(clf_batch* b) {
     }
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     DevBuf* dev[] = {&b->blobs, &b->desc, &b->order, &b->queue, &b->cand, &b->used, &b->tracks,
-                     &b->tracklets, &b->ttf_a, &b->ttf_b, &b->weak, &b->results, &b->prefix,
+                     &b->ttf_a, &b->ttf_b, &b->weak, &b->results, &b->prefix,
                      &b->packed};
     for (DevBuf* d : dev) d->release();
     b->h_blobs.release();
@@ -424,10 +423,9 @@ clf_status clf_batch_create(clf_context* ctx, int32_t n, const uint8_t* const* e
     chk(b->cand.ensure(sizeof(int4) * (size_t)hits));
     chk(b->used.ensure((size_t)hits));
     chk(b->tracks.ensure(sizeof(clf_track) * (size_t)cap));
-    chk(b->tracklets.ensure(sizeof(uint4) * (size_t)cap));
     chk(b->ttf_a.ensure(sizeof(TtfEntry) * (size_t)cap));
     chk(b->ttf_b.ensure(sizeof(TtfEntry) * (size_t)cap));
-    chk(b->weak.ensure(sizeof(uint32_t) * (size_t)cap));
+    chk(b->weak.ensure(sizeof(uint4) * (size_t)cap));
     chk(b->results.ensure(sizeof(EventResult) * (size_t)n));
     chk(b->prefix.ensure(sizeof(int64_t) * ((size_t)n + 1)));
     chk(b->packed.ensure(sizeof(clf_track) * (size_t)cap));

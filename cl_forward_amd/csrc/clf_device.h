@@ -31,13 +31,26 @@ struct EventResult {
 
 // A track or tracklet to follow.  "word" is the reference's ttf word
 // (bit 31 tracklet, bits 28-30 skipped modules, bits 0-27 index,
-// Kernel.cl:222-225); the last two hits' coordinates ride along so that
-// forwarding never has to gather them again.
+// Kernel.cl:222-225).  The last two hits' coordinates ride along so that
+// forwarding never gathers them again, and so do a tracklet's three hit
+// indices, so that no tracklet record has to be read back.
 struct TtfEntry {
     uint32_t word;
     uint32_t nhits;
-    float x0, y0, z0;  // second-to-last hit
-    float x1, y1, z1;  // last hit
+    uint32_t t0, t1, t2;  // tracklet hits (nhits == 3)
+    float x0, y0, z0;     // second-to-last hit
+    float x1, y1, z1;     // last hit
+    uint32_t pad;
+};
+
+// Candidate windows of one hit, relative to the start of the sensor they
+// point into, -1 = none.  W1Rec: the hit as h0 (window on the sensor two
+// below); W2Rec: the hit as h1.
+struct W1Rec {
+    int16_t lo, hi;
+};
+struct W2Rec {
+    int16_t lo, hi;
 };
 
 struct SearchArgs {
@@ -48,13 +61,14 @@ struct SearchArgs {
     int32_t n_items;
     int32_t* queue_head;   // work-queue counter, zero before the launch
     int32_t grain;         // events taken per dequeue
-    int4* cand;            // per hit: h1 window (x,y), h2 window (z,w)
-    uint8_t* used;         // per hit
+    int4* cand;            // per hit, 16 B: stage (a) output of the fill-only
+                           // check; otherwise W1Rec[N] then W2Rec[N] of
+                           // sensors too large for an LDS slot
+    uint8_t* used;         // per hit (events above the LDS bitmask)
     clf_track* tracks;     // per event, track_off records (the output)
-    uint4* tracklets;      // per event, track_off records: hits in x,y,z
     TtfEntry* ttf_a;       // per event, track_off entries
     TtfEntry* ttf_b;
-    uint32_t* weak;        // per event, track_off entries
+    uint4* weak;           // per event, track_off entries: tracklet hits
     EventResult* results;  // per event
     int32_t fill_only;     // stop after the candidate windows (stage (a) check)
     // Optional per-event phase profile (nullptr: off): kProfSlots counters per

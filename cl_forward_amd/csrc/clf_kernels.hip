@@ -2,31 +2,33 @@
 //
 // One workgroup owns one event at a time and pulls events from an atomic
 // work queue (persistent kernel).  Inside an event the sensor steps
-// f = S-1 .. 4 run in sequence (the reference's Kernel.cl:671-767 chain):
-// each step is forwarding, then the h0 gather, then seeding, each spread
-// over the workgroup.
+// f = S-1 .. 4 run in sequence (the reference's Kernel.cl:671-767 chain).
+// Each step is forwarding, then the h0 gather, then seeding, spread over
+// the workgroup; the candidate windows of the next step, which do not
+// depend on the used flags, are computed beside the forwarding by the
+// threads it leaves idle.
 //
-// Data placement.  A step touches three sensors: f (h0), f-2 (h1) and f-4
-// (h2, and the forwarding target).  Their hit coordinates live in a ring of
-// five LDS sensor slots (f .. f-4): every step stages one new sensor, f-4,
-// with coalesced loads.  A sensor with more hits than a slot holds (C, a
-// template parameter the autotuner picks) is read from HBM instead through
-// the same generic pointers.  The used flags are an LDS bitmask.  Only the
-// ttf lists, tracklets and output tracks go to HBM.
+// Data placement.  Step f reads hits of sensors f (h0), f-2 (h1), f-4 (h2
+// and the forwarding target) and prepares sensors f-1, f-3 and f-5 for the
+// next step, so the hit coordinates of sensors f .. f-5 sit in a ring of six
+// LDS slots; every step stages one new sensor.  A sensor with more hits
+// than a slot holds (C, a template parameter the autotuner picks) is read
+// from HBM instead.  Candidate windows live in small LDS double buffers
+// (HBM for oversized sensors), the used flags in an LDS bitmask.  Only the ttf lists, weak entries and output
+// tracks go to HBM.
 //
 // Stages:
 //   (a) candidate windows  Kernel.cl:50-178   window_h1 / window_h2: binary
-//       search on x-sorted sensor slices in LDS, exact scans otherwise;
-//       h2 windows of sensor f-2 at every step, h1 windows per gathered h0
+//       search on x-sorted slices, exact scans for unsorted sensors
 //   (c) forwarding         Kernel.cl:199-368  best h2 on sensor f-4 over the
 //       x-range a 0.6 mm tolerance allows
 //   (b) gather + seeding   Kernel.cl:392-568, 720-754  the (64,4) semantics:
 //       64-wide h0 chunks, 4 rows of h1 with the per-row stop
 //   (d) used flags, weak   Kernel.cl:313-363, 769-815
 //
-// Every append (tracks, tracklets, ttf entries, weak entries) goes through a
-// wave ballot / prefix sum in a fixed order instead of a global atomic, so
-// the output, order included, is the same on every run.
+// Every append (tracks, ttf entries, weak entries) goes through a wave
+// ballot / prefix sum in a fixed order instead of a global atomic, so the
+// output, order included, is the same on every run.
 //
 // FP32, built with -ffp-contract=off: every expression keeps the reference's
 // operation order.  Restricting a scan to an x-range is exact: it drops only
@@ -44,23 +46,24 @@ namespace clf {
 
 constexpr int kWave = 64;
 constexpr int kMaxVisits = 128;   // gathered h0 handled per seeding batch (multiple of 64)
-constexpr int kRing = 5;          // LDS sensor slots: f .. f-4
+constexpr int kRing = 6;          // LDS sensor slots: f .. f-5
 constexpr int kUsedBits = 32768;  // events up to this many hits keep used flags in LDS
 constexpr float kLim = 1.0e5f;    // |coordinate| bound of the supported domain
+constexpr int16_t kNoStop = 0x7fff;
 
 // Phase profile slots (wall-clock ticks, then work counters), filled when
 // SearchArgs::prof is set.  Thread 0 takes a timestamp after the barriers
 // that close each phase.
 enum ProfSlot {
-    PROF_PROLOGUE = 0,   // header, per-sensor checks, ring set-up
-    PROF_STAGE = 1,      // staging sensor f-4 into the ring
-    PROF_FORWARD = 2,    // h2 windows of sensor f-2 + forwarding
-    PROF_GATHER = 3,     // the h0 gather (wave 0)
-    PROF_SEED_SETUP = 4, // h1 windows of the gathered h0, pair prefix
-    PROF_SEED_STOP = 5,  // row stop points
-    PROF_SEED_EVAL = 6,  // (h0, h1) pairs: h2 range and fits
-    PROF_SEED_EMIT = 7,  // tracklets + ttf entries
-    PROF_TAIL = 8,       // tail + weak filter
+    PROF_PROLOGUE = 0,    // header, per-sensor checks, ring set-up
+    PROF_STAGE = 1,       // staging sensor f-5 into the ring
+    PROF_FORWARD = 2,     // forwarding + windows of the next step
+    PROF_GATHER = 3,      // the h0 gather (wave 0)
+    PROF_SEED_SETUP = 4,  // per-h0 window lookup, row stops, pair prefix
+    PROF_SEED_STOP = 5,   // (unused)
+    PROF_SEED_EVAL = 6,   // (h0, h1) pairs: h2 range and fits
+    PROF_SEED_EMIT = 7,   // ttf entries of the new tracklets
+    PROF_TAIL = 8,        // tail + weak filter
     PROF_TOTAL = 9,
     CNT_FWD_ENTRIES = 10,
     CNT_VISITS = 11,
@@ -119,12 +122,14 @@ struct Shared {
     float s_zmin[CLF_MAX_SENSORS];
     float s_zmax[CLF_MAX_SENSORS];
     int32_t s_sorted[CLF_MAX_SENSORS];
-    // ring of sensor slots: hit coordinates of sensors f .. f-4
+    // ring of sensor slots: hit coordinates of sensors f .. f-5
     float rx[kRing][C];
     float ry[kRing][C];
     float rz[kRing][C];
-    // h2 windows of the hits of sensor f-2, relative to sensor f-4's start
-    int2 w2[C];
+    // windows: sensor g's h1 windows in w1[g & 1], its h2 windows in
+    // w2[g & 1] (each buffer lives from its write to its last read)
+    W1Rec w1[2][C];
+    W2Rec w2[2][C];
     uint32_t used_bits[kUsedBits / 32];
     int32_t scan[3][W];
     int32_t item;
@@ -137,10 +142,9 @@ struct Shared {
     // the current batch of gathered h0 (indices relative to sensor f)
     int32_t v_h0[kMaxVisits];
     int32_t v_first[kMaxVisits];  // h1 window start, relative to sensor f-2
-    int32_t v_nh[kMaxVisits];
     int32_t v_pre[kMaxVisits + 1];
     float v_x0[kMaxVisits], v_y0[kMaxVisits], v_z0[kMaxVisits], v_dymax[kMaxVisits];
-    int32_t v_death[kMaxVisits * CLF_SEED_ROWS];
+    int16_t v_stop[kMaxVisits * CLF_SEED_ROWS];
     unsigned long long v_best[kMaxVisits];
     unsigned long long prof[kProfSlots];
 };
@@ -269,18 +273,22 @@ struct EventView {
     const float* X;
     const float* Y;
     const float* Zh;
+    W1Rec* hbm_w1;  // windows of sensors larger than a slot, per hit
+    W2Rec* hbm_w2;
 };
 
-template <int B, int C>
-__device__ __forceinline__ SView sensor_view(Shared<B, C>& sh, const EventView& ev, int s,
-                                             bool allow_lds) {
+// L = true: the caller knows sensor s sits in its ring slot, so the pointers
+// are LDS pointers the compiler can address with ds_read; otherwise the slot
+// or HBM is chosen at run time (generic addressing).
+template <bool L, int B, int C>
+__device__ __forceinline__ SView sensor_view(Shared<B, C>& sh, const EventView& ev, int s) {
     SView v;
     v.start = sh.s_start[s];
     v.n = sh.s_num[s];
     v.sorted = sh.s_sorted[s] != 0;
     v.zmin = sh.s_zmin[s];
     v.zmax = sh.s_zmax[s];
-    if (allow_lds && v.n <= C) {
+    if (L || v.n <= C) {
         const int k = s % kRing;
         v.x = sh.rx[k];
         v.y = sh.ry[k];
@@ -291,6 +299,29 @@ __device__ __forceinline__ SView sensor_view(Shared<B, C>& sh, const EventView& 
         v.z = ev.Zh + v.start;
     }
     return v;
+}
+
+template <int B, int C>
+__device__ __forceinline__ SView hbm_view(Shared<B, C>& sh, const EventView& ev, int s) {
+    SView v;
+    v.start = sh.s_start[s];
+    v.n = sh.s_num[s];
+    v.sorted = sh.s_sorted[s] != 0;
+    v.zmin = sh.s_zmin[s];
+    v.zmax = sh.s_zmax[s];
+    v.x = ev.X + v.start;
+    v.y = ev.Y + v.start;
+    v.z = ev.Zh + v.start;
+    return v;
+}
+
+template <bool L, int B, int C>
+__device__ __forceinline__ W1Rec* w1_of(Shared<B, C>& sh, const EventView& ev, int s) {
+    return (L || sh.s_num[s] <= C) ? sh.w1[s & 1] : ev.hbm_w1 + sh.s_start[s];
+}
+template <bool L, int B, int C>
+__device__ __forceinline__ W2Rec* w2_of(Shared<B, C>& sh, const EventView& ev, int s) {
+    return (L || sh.s_num[s] <= C) ? sh.w2[s & 1] : ev.hbm_w2 + sh.s_start[s];
 }
 
 // ---------------------------------------------------------------------------
@@ -366,6 +397,31 @@ __device__ int2 window_h2(const SView& s2, float x0, float z0, int zs0, int zs2)
         w.y = i;
     }
     return w;
+}
+
+// Precomputation for a step, one item per hit:
+//  items [0, n_w2):        h2 windows of sensor g2's hits into g2-2
+//  items [n_w2, n_w2+n_w1): h1 windows of sensor g1's hits into g1-2
+template <bool L, int B, int C>
+__device__ __forceinline__ void window_item(Shared<B, C>& sh, const EventView& ev, int i, int g2,
+                                            int n_w2, int g1) {
+    if (i < n_w2) {
+        const SView src = sensor_view<L>(sh, ev, g2);
+        const SView dst = sensor_view<L>(sh, ev, g2 - 2);
+        const int2 w = window_h2(dst, src.x[i], src.z[i], sh.s_Z[g2 + 2], sh.s_Z[g2 - 2]);
+        W2Rec r;
+        r.lo = (int16_t)w.x;
+        r.hi = (int16_t)w.y;
+        w2_of<L>(sh, ev, g2)[i] = r;
+    } else {
+        i -= n_w2;
+        const SView src = sensor_view<L>(sh, ev, g1);
+        const SView dst = sensor_view<L>(sh, ev, g1 - 2);
+        const int2 w = window_h1(dst, src.x[i], src.z[i]);
+        W1Rec* r = w1_of<L>(sh, ev, g1) + i;
+        r->lo = (int16_t)w.x;
+        r->hi = (int16_t)w.y;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -446,9 +502,10 @@ __device__ void gather_batch(Shared<B, C>& sh, const UsedFlags& used, int start0
 struct StepCtx {
     int f;
     SView v0, v2, v4;  // sensors f, f-2, f-4
-    const int2* w2;    // h2 windows of sensor f-2 (relative to f-4)
+    const W1Rec* w1;   // h1 windows + row stops of sensor f
+    const W2Rec* w2;   // h2 windows of sensor f-2 (relative to f-4)
     bool fast4;        // sensor f-4 x-sorted and the event in the supported domain
-    bool seeds;        // sensors f-2 >= 4 have h2 windows
+    bool seeds;        // h1 sensor f-2 >= 4 has h2 windows (Kernel.cl:58,104)
 };
 
 // ---------------------------------------------------------------------------
@@ -460,8 +517,7 @@ struct StepCtx {
 // ---------------------------------------------------------------------------
 template <int B, int C>
 __device__ void seed_batch(Shared<B, C>& sh, const StepCtx& sc, const UsedFlags& used,
-                           uint4* tracklets, TtfEntry* nxt, int nv, int& n_next, int& n_tracklets,
-                           PhaseClock& pc) {
+                           TtfEntry* nxt, int nv, int& n_next, int& n_tracklets, PhaseClock& pc) {
     const int tid = threadIdx.x;
     const SView& v0 = sc.v0;
     const SView& v2 = sc.v2;
@@ -469,18 +525,51 @@ __device__ void seed_batch(Shared<B, C>& sh, const StepCtx& sc, const UsedFlags&
     // z of the first hit of sensor f-2 (:420-422)
     const float s1_z = v2.n > 0 ? v2.z[0] : 0.f;
 
+    // per h0: window, row stops, and the part of the window a live row can
+    // reach.  Row y takes window elements 4j+y and stops for good at the
+    // first whose h1 has no h2 window (Kernel.cl:436-457); without h2
+    // windows on sensor f-4 every row stops at once.  The h2 windows are
+    // read eight at a time so the loads overlap.
     for (int v = tid; v < nv; v += B) {
         const int e0 = sh.v_h0[v];
-        const float x0 = v0.x[e0], z0 = v0.z[e0];
-        const int2 w = window_h1(v2, x0, z0);
-        sh.v_first[v] = w.x;
-        sh.v_nh[v] = (int)(uint32_t)(w.y - w.x);
-        sh.v_x0[v] = x0;
+        const W1Rec r = sc.w1[e0];
+        const int nh = r.lo >= 0 ? r.hi - r.lo : 0;
+        int16_t d[4] = {kNoStop, kNoStop, kNoStop, kNoStop};
+        if (!sc.seeds) {
+            d[0] = d[1] = d[2] = d[3] = 0;
+        } else {
+            const W2Rec* w2 = sc.w2 + r.lo;
+            int found = 0;
+            for (int b0 = 0; b0 < nh && found < 4; b0 += 8) {
+                // unconditional loads (index clamped into the window) so
+                // that all eight are in flight together
+                int16_t lo[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) lo[k] = w2[min(b0 + k, nh - 1)].lo;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const int el = b0 + k;
+                    if (el < nh && lo[k] < 0 && d[el & 3] == kNoStop) {
+                        d[el & 3] = (int16_t)(el >> 2);
+                        ++found;
+                    }
+                }
+            }
+        }
+        int reach = 0;
+#pragma unroll
+        for (int y = 0; y < CLF_SEED_ROWS; ++y) {
+            sh.v_stop[v * CLF_SEED_ROWS + y] = d[y];
+            const int lim = d[y] == kNoStop ? nh : min(nh, CLF_SEED_ROWS * (int)d[y] + y);
+            reach = max(reach, lim);
+        }
+        sh.v_first[v] = r.lo;
+        sh.v_pre[v] = reach;  // counts for now, prefix below
+        const float z0 = v0.z[e0];
+        sh.v_x0[v] = v0.x[e0];
         sh.v_y0[v] = v0.y[e0];
         sh.v_z0[v] = z0;
         sh.v_dymax[v] = CLF_PARAM_MAXYSLOPE * fabsf(s1_z - z0);
-#pragma unroll
-        for (int y = 0; y < CLF_SEED_ROWS; ++y) sh.v_death[v * CLF_SEED_ROWS + y] = 0x7fffffff;
         sh.v_best[v] = ~0ull;
     }
     __syncthreads();
@@ -489,7 +578,7 @@ __device__ void seed_batch(Shared<B, C>& sh, const StepCtx& sc, const UsedFlags&
         int running = 0;
         for (int c0 = 0; c0 < nv; c0 += kWave) {
             const int v = c0 + lane;
-            const int x = v < nv ? sh.v_nh[v] : 0;
+            const int x = v < nv ? sh.v_pre[v] : 0;
             const int incl = wave_incl_scan(x);
             if (v < nv) sh.v_pre[v] = running + incl - x;
             running += __shfl(incl, kWave - 1);
@@ -501,105 +590,96 @@ __device__ void seed_batch(Shared<B, C>& sh, const StepCtx& sc, const UsedFlags&
     prof_mark(sh, pc, PROF_SEED_SETUP);
     prof_add(sh, pc, CNT_PAIRS, (unsigned long long)P);
 
-    auto visit_of = [&](int p) {  // last v with v_pre[v] <= p
-        int lo = 0, hi = nv;
+    // pairs (visit v, window element e), p = v_pre[v] + e; each thread walks
+    // its pairs in increasing p, so its visit index only moves forward
+    int v = 0;
+    if (tid < P) {
+        int lo = 0, hi = nv;  // last v with v_pre[v] <= tid
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
-            if (sh.v_pre[mid] <= p) lo = mid;
+            if (sh.v_pre[mid] <= tid) lo = mid;
             else hi = mid;
         }
-        return lo;
-    };
-
-    // row stop points: first j of row y whose h1 has no h2 window (:453-457)
-    for (int p = tid; p < P; p += B) {
-        const int v = visit_of(p);
-        const int e = p - sh.v_pre[v];
-        const int h1 = sh.v_first[v] + e;
-        if (!sc.seeds || sc.w2[h1].x == -1)
-            atomicMin(&sh.v_death[v * CLF_SEED_ROWS + (e & (CLF_SEED_ROWS - 1))], e / CLF_SEED_ROWS);
+        v = lo;
     }
-    __syncthreads();
-    prof_mark(sh, pc, PROF_SEED_STOP);
-
-    if (sc.seeds) {
-        for (int p = tid; p < P; p += B) {
-            const int v = visit_of(p);
-            const int e = p - sh.v_pre[v];
-            const int y = e & (CLF_SEED_ROWS - 1);
-            const int j = e / CLF_SEED_ROWS;
-            if (j >= sh.v_death[v * CLF_SEED_ROWS + y]) continue;
-            const int h1 = sh.v_first[v] + e;
-            if (used.get(v2.start + h1)) continue;
-            const float x0 = sh.v_x0[v], y0 = sh.v_y0[v], z0 = sh.v_z0[v];
-            const float x1 = v2.x[h1], y1 = v2.y[h1], z1 = v2.z[h1];
-            if (!(fabsf(y1 - y0) < sh.v_dymax[v])) continue;  // :509, independent of h2
-            const int2 c = sc.w2[h1];
-            const float dz_inverted = 1.f / (z1 - z0);
-            int k0 = c.x, k1 = c.y;
-            if (sc.fast4) {
-                const float pa = x0 + (x1 - x0) * ((v4.zmin - z0) * dz_inverted);
-                const float pb = x0 + (x1 - x0) * ((v4.zmax - z0) * dz_inverted);
-                float lo, hi;
-                x_window(pa, pb, lo, hi);
-                k0 = lb_ge(v4.x, k0, k1, lo);
-                k1 = ub_gt(v4.x, k0, k1, hi);
-            }
-            float best = FLT_MAX;
-            int bk = -1;
-            for (int k = k0; k < k1; ++k) {
-                const float x2 = v4.x[k], y2 = v4.y[k], z2 = v4.z[k];
-                const float z2_tz = (z2 - z0) * dz_inverted;
-                const float x = x0 + (x1 - x0) * z2_tz;
-                const float yy = y0 + (y1 - y0) * z2_tz;
-                const float dx = x - x2;
-                const float dy = yy - y2;
-                if (fabsf(dx) < CLF_PARAM_TOLERANCE && fabsf(dy) < CLF_PARAM_TOLERANCE) {
-                    const float scatterNum = (dx * dx) + (dy * dy);
-                    const float scatterDenom = 1.f / (z2 - z1);
-                    const float scatter = scatterNum * scatterDenom * scatterDenom;
-                    if (scatter < CLF_MAX_SCATTER && scatter < best) {
-                        best = scatter;
-                        bk = k;
-                    }
+    for (int p = tid; p < P; p += B) {
+        while (sh.v_pre[v + 1] <= p) ++v;
+        const int e = p - sh.v_pre[v];
+        const int y = e & (CLF_SEED_ROWS - 1);
+        const int j = e / CLF_SEED_ROWS;
+        if (j >= sh.v_stop[v * CLF_SEED_ROWS + y]) continue;
+        const int h1 = sh.v_first[v] + e;
+        if (used.get(v2.start + h1)) continue;
+        const float x0 = sh.v_x0[v], y0 = sh.v_y0[v], z0 = sh.v_z0[v];
+        const float x1 = v2.x[h1], y1 = v2.y[h1], z1 = v2.z[h1];
+        if (!(fabsf(y1 - y0) < sh.v_dymax[v])) continue;  // :509, independent of h2
+        const W2Rec c = sc.w2[h1];
+        const float dz_inverted = 1.f / (z1 - z0);
+        int k = c.lo;
+        const int k1 = c.hi;
+        float hi = INFINITY;
+        if (sc.fast4) {
+            const float pa = x0 + (x1 - x0) * ((v4.zmin - z0) * dz_inverted);
+            const float pb = x0 + (x1 - x0) * ((v4.zmax - z0) * dz_inverted);
+            float lo;
+            x_window(pa, pb, lo, hi);
+            k = lb_ge(v4.x, k, k1, lo);
+        }
+        float best = FLT_MAX;
+        int bk = -1;
+        for (; k < k1; ++k) {
+            const float x2 = v4.x[k], y2 = v4.y[k], z2 = v4.z[k];
+            if (x2 > hi) break;  // sorted: nothing further can pass
+            const float z2_tz = (z2 - z0) * dz_inverted;
+            const float x = x0 + (x1 - x0) * z2_tz;
+            const float yy = y0 + (y1 - y0) * z2_tz;
+            const float dx = x - x2;
+            const float dy = yy - y2;
+            if (fabsf(dx) < CLF_PARAM_TOLERANCE && fabsf(dy) < CLF_PARAM_TOLERANCE) {
+                const float scatterNum = (dx * dx) + (dy * dy);
+                const float scatterDenom = 1.f / (z2 - z1);
+                const float scatter = scatterNum * scatterDenom * scatterDenom;
+                if (scatter < CLF_MAX_SCATTER && scatter < best) {
+                    best = scatter;
+                    bk = k;
                 }
             }
-            if (bk >= 0) {
-                const unsigned long long key =
-                    ((unsigned long long)__float_as_uint(best + 0.0f) << 32) |
-                    ((unsigned long long)y << 30) | ((unsigned long long)j << 16) |
-                    (unsigned long long)bk;
-                atomicMin(&sh.v_best[v], key);
-            }
+        }
+        if (bk >= 0) {
+            const unsigned long long key = ((unsigned long long)__float_as_uint(best + 0.0f) << 32) |
+                                           ((unsigned long long)y << 30) |
+                                           ((unsigned long long)j << 16) | (unsigned long long)bk;
+            atomicMin(&sh.v_best[v], key);
         }
     }
     __syncthreads();
     prof_mark(sh, pc, PROF_SEED_EVAL);
 
-    // tracklets and their ttf entries, in visit order (:550-566)
+    // tracklets, as ttf entries, in visit order (:550-566)
     for (int base = 0; base < nv; base += B) {
-        const int v = base + tid;
-        const bool acc = v < nv && sh.v_best[v] != ~0ull;
+        const int vv = base + tid;
+        const bool acc = vv < nv && sh.v_best[vv] != ~0ull;
         int pos[3], tot[3];
         block_scan3<B, C>(sh, acc, false, false, pos, tot);
         if (acc) {
-            const unsigned long long key = sh.v_best[v];
+            const unsigned long long key = sh.v_best[vv];
             const int y = (int)((key >> 30) & 3ull);
             const int j = (int)((key >> 16) & 0x3fffull);
             const int k = (int)(key & 0xffffull);
-            const int h1 = sh.v_first[v] + CLF_SEED_ROWS * j + y;
-            const int idx = n_tracklets + pos[0];
-            tracklets[idx] = make_uint4((uint32_t)(v0.start + sh.v_h0[v]), (uint32_t)(v2.start + h1),
-                                        (uint32_t)(v4.start + k), 0u);
+            const int h1 = sh.v_first[vv] + CLF_SEED_ROWS * j + y;
             TtfEntry t;
-            t.word = CLF_TTF_TRACKLET_BIT | (uint32_t)idx;
+            t.word = CLF_TTF_TRACKLET_BIT | (uint32_t)(n_tracklets + pos[0]);
             t.nhits = 3;
+            t.t0 = (uint32_t)(v0.start + sh.v_h0[vv]);
+            t.t1 = (uint32_t)(v2.start + h1);
+            t.t2 = (uint32_t)(v4.start + k);
             t.x0 = v2.x[h1];
             t.y0 = v2.y[h1];
             t.z0 = v2.z[h1];
             t.x1 = v4.x[k];
             t.y1 = v4.y[k];
             t.z1 = v4.z[k];
+            t.pad = 0;
             nxt[n_next + pos[0]] = t;
         }
         n_tracklets += tot[0];
@@ -620,6 +700,201 @@ __device__ __forceinline__ void stage_sensor(Shared<B, C>& sh, const EventView& 
         sh.ry[k][i] = ev.Y[b + i];
         sh.rz[k][i] = ev.Zh[b + i];
     }
+}
+
+// Number of window items for the step whose h1 sensor is g2 and h0 sensor g1.
+template <int B, int C>
+__device__ __forceinline__ int n_w2_items(const Shared<B, C>& sh, int g2) {
+    return g2 >= 4 ? sh.s_num[g2] : 0;
+}
+template <int B, int C>
+__device__ __forceinline__ int n_w1_items(const Shared<B, C>& sh, int g1) {
+    return g1 >= 4 ? sh.s_num[g1] : 0;
+}
+
+// Sensor-loop state of one event (Kernel.cl:669-767).
+struct StepState {
+    int n_cur, n_tracks, n_tracklets, n_weak, visits;
+    TtfEntry* cur;  // entries inserted in the previous step
+    TtfEntry* nxt;  // entries inserted in this step
+};
+
+// One sensor step f: forwarding, gather, seeding, plus the windows and row
+// stops of step f-1.  L: every sensor f .. f-5 sits in the LDS ring.
+template <bool L, int B, int C>
+__device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
+                                         const UsedFlags& used, clf_track* tracks, uint4* weak,
+                                         bool degenerate, int f, StepState& st, PhaseClock& pc) {
+    constexpr int W = Shared<B, C>::W;
+    const int tid = threadIdx.x, wave = wave_id();
+    int& n_cur = st.n_cur;
+    int& n_tracks = st.n_tracks;
+    int& n_tracklets = st.n_tracklets;
+    int& n_weak = st.n_weak;
+    int& visits = st.visits;
+    TtfEntry*& cur = st.cur;
+    TtfEntry*& nxt = st.nxt;
+    StepCtx sc;
+    sc.f = f;
+    sc.v0 = sensor_view<L>(sh, ev, f);
+    sc.v2 = sensor_view<L>(sh, ev, f - 2);
+    sc.v4 = sensor_view<L>(sh, ev, f - 4);
+    sc.w1 = w1_of<L>(sh, ev, f);
+    sc.w2 = w2_of<L>(sh, ev, f - 2);
+    sc.fast4 = sc.v4.sorted && !degenerate;
+    sc.seeds = f - 2 >= 4;
+    const SView& v4 = sc.v4;
+
+    // windows of the next step (h2 windows of f-3, h1 windows of f-1),
+    // on the threads the forwarding leaves free (taken from the top)
+    {
+        const int g2 = f - 3, g1 = f - 1;
+        const int nw2 = g2 >= 4 && f - 1 >= 4 ? n_w2_items(sh, g2) : 0;
+        const int nw1 = n_w1_items(sh, g1);
+        for (int i = B - 1 - tid; i < nw2 + nw1; i += B) window_item<L>(sh, ev, i, g2, nw2, g1);
+    }
+
+    // (c) forwarding of the ttf entries of step f+1 (Kernel.cl:210-367)
+    int n_next = 0;
+    for (int base = 0; base < n_cur; base += B) {
+        const int i = base + tid;
+        const bool active = i < n_cur;
+        TtfEntry t = {};
+        bool found = false, f_new = false, f_req = false, f_weak = false;
+        int bk = -1;
+        if (active) {
+            t = cur[i];
+            const bool is_tl = (t.word & CLF_TTF_TRACKLET_BIT) != 0;
+            const uint32_t skipped = (t.word & CLF_TTF_SKIP_MASK) >> CLF_TTF_SKIP_SHIFT;
+            const float td = 1.0f / (t.z1 - t.z0);
+            const float txn = (t.x1 - t.x0);
+            const float tyn = (t.y1 - t.y0);
+            const float tx = txn * td;
+            const float ty = tyn * td;
+            int k = 0;
+            float hi = INFINITY;
+            if (sc.fast4 && v4.n > 0) {
+                const float pa = t.x0 + tx * (v4.zmin - t.z0);
+                const float pb = t.x0 + tx * (v4.zmax - t.z0);
+                float lo;
+                x_window(pa, pb, lo, hi);
+                k = lb_ge(v4.x, 0, v4.n, lo);
+            }
+            float best_fit = FLT_MAX;
+            for (; k < v4.n; ++k) {
+                const float x2 = v4.x[k], y2 = v4.y[k], z2 = v4.z[k];
+                if (x2 > hi) break;  // sorted: nothing further can pass
+                const float fit = fit_hit_to_track(tx, ty, t.x0, t.y0, t.z0, t.z1, x2, y2, z2);
+                const bool better = fit < best_fit;
+                best_fit = (float)better * fit + (float)(!better) * best_fit;
+                bk = better ? k : bk;
+            }
+            if (best_fit != FLT_MAX) {
+                if (bk >= 0) {
+                    found = true;
+                    f_new = is_tl;
+                    f_req = t.nhits + 1 < CLF_MAX_TRACK_SIZE;
+                } else {
+                    // NaN best with no hit recorded: only in the
+                    // degenerate domain; the track is dropped
+                    atomicOr(&sh.flags, CLF_FLAG_DEGENERATE_Z);
+                }
+            } else if (skipped <= CLF_MAX_SKIPPED_MODULES) {
+                f_req = true;
+            } else if (t.nhits == 3) {
+                f_weak = true;
+            }
+        }
+        int pos[3], tot[3];
+        block_scan3<B, C>(sh, f_new, f_req, f_weak, pos, tot);
+        if (active) {
+            if (found) {
+                const int b = v4.start + bk;
+                used.set(b);
+                uint32_t slot;
+                if (f_new) {
+                    used.set((int)t.t0);
+                    used.set((int)t.t1);
+                    used.set((int)t.t2);
+                    slot = (uint32_t)(n_tracks + pos[0]);
+                    uint32_t* r = reinterpret_cast<uint32_t*>(tracks + slot);
+                    r[0] = 4;
+                    r[1] = t.t0;
+                    r[2] = t.t1;
+                    r[3] = t.t2;
+                    r[4] = (uint32_t)b;
+#pragma unroll
+                    for (int q = 5; q < 1 + CLF_MAX_TRACK_SIZE; ++q) r[q] = 0;
+                } else {
+                    slot = t.word & CLF_TTF_INDEX_MASK;
+                    tracks[slot].hits[t.nhits] = (uint32_t)b;
+                    tracks[slot].hitsNum = t.nhits + 1;
+                }
+                if (f_req) {
+                    TtfEntry n;
+                    n.word = slot;
+                    n.nhits = t.nhits + 1;
+                    n.t0 = n.t1 = n.t2 = 0;
+                    n.x0 = t.x1;
+                    n.y0 = t.y1;
+                    n.z0 = t.z1;
+                    n.x1 = v4.x[bk];
+                    n.y1 = v4.y[bk];
+                    n.z1 = v4.z[bk];
+                    n.pad = 0;
+                    nxt[n_next + pos[1]] = n;
+                } else {
+                    atomicOr(&sh.flags, CLF_FLAG_TRACK_FULL);
+                    atomicAdd(&sh.n_frozen, 1);
+                }
+            } else if (f_req) {
+                const uint32_t skipped = (t.word & CLF_TTF_SKIP_MASK) >> CLF_TTF_SKIP_SHIFT;
+                TtfEntry n = t;
+                n.word = ((skipped + 1) << CLF_TTF_SKIP_SHIFT) | (t.word & CLF_TTF_KEEP_MASK);
+                nxt[n_next + pos[1]] = n;
+            } else if (f_weak) {
+                weak[n_weak + pos[2]] = make_uint4(t.t0, t.t1, t.t2, 0u);
+            }
+        }
+        n_tracks += tot[0];
+        n_next += tot[1];
+        n_weak += tot[2];
+    }
+    if (tid == 0) {
+        sh.g_prev = 0;
+        sh.g_shift = CLF_GATHER_CHUNK;
+        sh.g_counter = 0;
+    }
+    __syncthreads();  // used flags and windows visible
+    prof_mark(sh, pc, PROF_FORWARD);
+
+    // (b) gather + seeding (Kernel.cl:720-764)
+    const int start0 = sc.v0.start;
+    const uint32_t hn0 = (uint32_t)sc.v0.n;
+    for (;;) {
+        if (wave == 0) gather_batch<B, C>(sh, used, start0, hn0);
+        __syncthreads();
+        prof_mark(sh, pc, PROF_GATHER);
+        const int nv = sh.g_nv;
+        const bool more = sh.g_more != 0;
+        visits += nv;
+        prof_add(sh, pc, CNT_VISITS, (unsigned long long)nv);
+        prof_add(sh, pc, CNT_BATCHES, 1ull);
+        if (nv > 0) seed_batch<B, C>(sh, sc, used, nxt, nv, n_next, n_tracklets, pc);
+        __syncthreads();
+        prof_mark(sh, pc, PROF_SEED_EMIT);
+        if (!more) break;
+    }
+
+    // span of the reference's ttf ring during this step
+    if (tid == 0) {
+        const int live = n_cur + n_next;
+        if (live > sh.max_live) sh.max_live = live;
+    }
+    TtfEntry* tmp = cur;
+    cur = nxt;
+    nxt = tmp;
+    n_cur = n_next;
 }
 
 // ---------------------------------------------------------------------------
@@ -658,11 +933,12 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
     ev.X = (const float*)(gNum + S + N);
     ev.Y = ev.X + N;
     ev.Zh = ev.Y + N;
+    ev.hbm_w1 = reinterpret_cast<W1Rec*>(a.cand + d.hit_off);
+    ev.hbm_w2 = reinterpret_cast<W2Rec*>(ev.hbm_w1 + N);
     clf_track* tracks = a.tracks + d.track_off;
-    uint4* tracklets = a.tracklets + d.track_off;
     TtfEntry* cur = a.ttf_a + d.track_off;
     TtfEntry* nxt = a.ttf_b + d.track_off;
-    uint32_t* weak = a.weak + d.track_off;
+    uint4* weak = a.weak + d.track_off;
     UsedFlags used;
     used.lds = N <= kUsedBits;
     used.bits = sh.used_bits;
@@ -726,6 +1002,8 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
         if (sh.s_num[f] > 0 && zs0 >= sh.s_zmin[f] && zs0 <= sh.s_zmax[f])
             atomicOr(&sh.flags, CLF_FLAG_DEGENERATE_Z);
     }
+    // the ring starts with sensors S-1 .. S-5
+    for (int s = S - 1; s >= S - 5 && s >= 0; --s) stage_sensor<B, C>(sh, ev, s);
     __syncthreads();
     const bool degenerate = (sh.flags & CLF_FLAG_DEGENERATE_Z) != 0;
 
@@ -736,7 +1014,7 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
         for (int i = tid; i < N; i += B) cand[i] = make_int4(-1, -1, -1, -1);
         __syncthreads();
         for (int f = 4; f < S; ++f) {
-            const SView v2 = sensor_view<B, C>(sh, ev, f - 2, false);
+            const SView v2 = hbm_view(sh, ev, f - 2);
             const int st = sh.s_start[f], n = sh.s_num[f];
             const bool p2 = f <= S - 3;
             for (int i = tid; i < n; i += B) {
@@ -756,187 +1034,51 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
         return;
     }
 
-    // the ring starts with sensors S-1 .. S-4
-    for (int s = S - 1; s >= S - 4 && s >= 0; --s) stage_sensor<B, C>(sh, ev, s);
+    // windows and row stops for the first step (f = S-1): h2 windows of
+    // sensor S-3, h1 windows of sensor S-1, then the row stops of S-1
+    if (S - 1 >= 4) {
+        const int nw2 = n_w2_items(sh, S - 3), nw1 = n_w1_items(sh, S - 1);
+        bool lds = true;
+        for (int s2 = S - 1; s2 >= S - 5 && s2 >= 0; --s2) lds = lds && sh.s_num[s2] <= C;
+        for (int i = tid; i < nw2 + nw1; i += B) {
+            if (lds) window_item<true>(sh, ev, i, S - 3, nw2, S - 1);
+            else window_item<false>(sh, ev, i, S - 3, nw2, S - 1);
+        }
+        __syncthreads();
+    }
     prof_mark(sh, pc, PROF_PROLOGUE);
 
     int n_cur = 0, n_tracks = 0, n_tracklets = 0, n_weak = 0, visits = 0;
-    int2* w2_hbm = reinterpret_cast<int2*>(a.cand + d.hit_off);  // h2 windows of oversized sensors
     for (int f = S - 1; f >= 4; --f) {
-        stage_sensor<B, C>(sh, ev, f - 4);
+        stage_sensor<B, C>(sh, ev, f - 5);
         __syncthreads();
         prof_mark(sh, pc, PROF_STAGE);
         prof_add(sh, pc, CNT_FWD_ENTRIES, (unsigned long long)n_cur);
         prof_add(sh, pc, CNT_FWD_TILES, (unsigned long long)((n_cur + B - 1) / B));
-        StepCtx sc;
-        sc.f = f;
-        sc.v0 = sensor_view<B, C>(sh, ev, f, true);
-        sc.v2 = sensor_view<B, C>(sh, ev, f - 2, true);
-        sc.v4 = sensor_view<B, C>(sh, ev, f - 4, true);
-        sc.fast4 = sc.v4.sorted && !degenerate;
-        sc.seeds = f - 2 >= 4;
-        int2* w2 = sc.v2.n <= C ? sh.w2 : w2_hbm + sc.v2.start;
-        sc.w2 = w2;
-        const SView& v4 = sc.v4;
-
-        // (a) h2 windows of sensor f-2's hits into sensor f-4
-        if (sc.seeds) {
-            const int zs0 = sh.s_Z[f], zs2 = sh.s_Z[f - 4];
-            for (int i = tid; i < sc.v2.n; i += B)
-                w2[i] = window_h2(v4, sc.v2.x[i], sc.v2.z[i], zs0, zs2);
-        }
-
-        // (c) forwarding of the ttf entries of step f+1 (Kernel.cl:210-367)
-        int n_next = 0;
-        for (int base = 0; base < n_cur; base += B) {
-            const int i = base + tid;
-            const bool active = i < n_cur;
-            TtfEntry t = {};
-            bool found = false, f_new = false, f_req = false, f_weak = false;
-            int bk = -1;
-            if (active) {
-                t = cur[i];
-                const bool is_tl = (t.word & CLF_TTF_TRACKLET_BIT) != 0;
-                const uint32_t skipped = (t.word & CLF_TTF_SKIP_MASK) >> CLF_TTF_SKIP_SHIFT;
-                const float td = 1.0f / (t.z1 - t.z0);
-                const float txn = (t.x1 - t.x0);
-                const float tyn = (t.y1 - t.y0);
-                const float tx = txn * td;
-                const float ty = tyn * td;
-                int k0 = 0, k1 = v4.n;
-                if (sc.fast4 && v4.n > 0) {
-                    const float pa = t.x0 + tx * (v4.zmin - t.z0);
-                    const float pb = t.x0 + tx * (v4.zmax - t.z0);
-                    float lo, hi;
-                    x_window(pa, pb, lo, hi);
-                    k0 = lb_ge(v4.x, 0, v4.n, lo);
-                    k1 = ub_gt(v4.x, k0, v4.n, hi);
-                }
-                float best_fit = FLT_MAX;
-                for (int k = k0; k < k1; ++k) {
-                    const float fit =
-                        fit_hit_to_track(tx, ty, t.x0, t.y0, t.z0, t.z1, v4.x[k], v4.y[k], v4.z[k]);
-                    const bool better = fit < best_fit;
-                    best_fit = (float)better * fit + (float)(!better) * best_fit;
-                    bk = better ? k : bk;
-                }
-                if (best_fit != FLT_MAX) {
-                    if (bk >= 0) {
-                        found = true;
-                        f_new = is_tl;
-                        f_req = t.nhits + 1 < CLF_MAX_TRACK_SIZE;
-                    } else {
-                        // NaN best with no hit recorded: only in the
-                        // degenerate domain; the track is dropped
-                        atomicOr(&sh.flags, CLF_FLAG_DEGENERATE_Z);
-                    }
-                } else if (skipped <= CLF_MAX_SKIPPED_MODULES) {
-                    f_req = true;
-                } else if (t.nhits == 3) {
-                    f_weak = true;
-                }
-            }
-            int pos[3], tot[3];
-            block_scan3<B, C>(sh, f_new, f_req, f_weak, pos, tot);
-            if (active) {
-                const uint32_t idx = t.word & CLF_TTF_INDEX_MASK;
-                if (found) {
-                    const int b = v4.start + bk;
-                    used.set(b);
-                    uint32_t slot;
-                    if (f_new) {
-                        const uint4 tl = tracklets[idx];
-                        used.set((int)tl.x);
-                        used.set((int)tl.y);
-                        used.set((int)tl.z);
-                        slot = (uint32_t)(n_tracks + pos[0]);
-                        uint32_t* r = reinterpret_cast<uint32_t*>(tracks + slot);
-                        r[0] = 4;
-                        r[1] = tl.x;
-                        r[2] = tl.y;
-                        r[3] = tl.z;
-                        r[4] = (uint32_t)b;
-#pragma unroll
-                        for (int q = 5; q < 1 + CLF_MAX_TRACK_SIZE; ++q) r[q] = 0;
-                    } else {
-                        slot = idx;
-                        tracks[slot].hits[t.nhits] = (uint32_t)b;
-                        tracks[slot].hitsNum = t.nhits + 1;
-                    }
-                    if (f_req) {
-                        TtfEntry n;
-                        n.word = slot;
-                        n.nhits = t.nhits + 1;
-                        n.x0 = t.x1;
-                        n.y0 = t.y1;
-                        n.z0 = t.z1;
-                        n.x1 = v4.x[bk];
-                        n.y1 = v4.y[bk];
-                        n.z1 = v4.z[bk];
-                        nxt[n_next + pos[1]] = n;
-                    } else {
-                        atomicOr(&sh.flags, CLF_FLAG_TRACK_FULL);
-                        atomicAdd(&sh.n_frozen, 1);
-                    }
-                } else if (f_req) {
-                    const uint32_t skipped = (t.word & CLF_TTF_SKIP_MASK) >> CLF_TTF_SKIP_SHIFT;
-                    TtfEntry n = t;
-                    n.word = ((skipped + 1) << CLF_TTF_SKIP_SHIFT) | (t.word & CLF_TTF_KEEP_MASK);
-                    nxt[n_next + pos[1]] = n;
-                } else if (f_weak) {
-                    weak[n_weak + pos[2]] = idx;
-                }
-            }
-            n_tracks += tot[0];
-            n_next += tot[1];
-            n_weak += tot[2];
-        }
-        if (tid == 0) {
-            sh.g_prev = 0;
-            sh.g_shift = CLF_GATHER_CHUNK;
-            sh.g_counter = 0;
-        }
-        __syncthreads();  // used flags and h2 windows of this step visible to seeding
-        prof_mark(sh, pc, PROF_FORWARD);
-
-        // (b) gather + seeding (Kernel.cl:720-764)
-        const int start0 = sc.v0.start;
-        const uint32_t hn0 = (uint32_t)sc.v0.n;
-        for (;;) {
-            if (wave == 0) gather_batch<B, C>(sh, used, start0, hn0);
-            __syncthreads();
-            prof_mark(sh, pc, PROF_GATHER);
-            const int nv = sh.g_nv;
-            const bool more = sh.g_more != 0;
-            visits += nv;
-            prof_add(sh, pc, CNT_VISITS, (unsigned long long)nv);
-            prof_add(sh, pc, CNT_BATCHES, 1ull);
-            if (nv > 0)
-                seed_batch<B, C>(sh, sc, used, tracklets, nxt, nv, n_next, n_tracklets, pc);
-            __syncthreads();
-            prof_mark(sh, pc, PROF_SEED_EMIT);
-            if (!more) break;
-        }
-
-        // span of the reference's ttf ring during this step
-        if (tid == 0) {
-            const int live = n_cur + n_next;
-            if (live > sh.max_live) sh.max_live = live;
-        }
-        TtfEntry* tmp = cur;
-        cur = nxt;
-        nxt = tmp;
-        n_cur = n_next;
+        StepState st = {n_cur, n_tracks, n_tracklets, n_weak, visits, cur, nxt};
+        bool lds_step = true;
+        for (int s2 = f; s2 >= f - 5 && s2 >= 0; --s2) lds_step = lds_step && sh.s_num[s2] <= C;
+        if (lds_step)
+            run_step<true>(sh, ev, used, tracks, weak, degenerate, f, st, pc);
+        else
+            run_step<false>(sh, ev, used, tracks, weak, degenerate, f, st, pc);
+        n_cur = st.n_cur;
+        n_tracks = st.n_tracks;
+        n_tracklets = st.n_tracklets;
+        n_weak = st.n_weak;
+        visits = st.visits;
+        cur = st.cur;
+        nxt = st.nxt;
     }
 
     // tail: tracklet entries of the last step become weak (Kernel.cl:771-792)
     for (int base = 0; base < n_cur; base += B) {
         const int i = base + tid;
-        uint32_t w = 0;
-        const bool tl = i < n_cur && ((w = cur[i].word) & CLF_TTF_TRACKLET_BIT) != 0;
+        TtfEntry t = {};
+        const bool tl = i < n_cur && ((t = cur[i]).word & CLF_TTF_TRACKLET_BIT) != 0;
         int pos[3], tot[3];
         block_scan3<B, C>(sh, tl, false, false, pos, tot);
-        if (tl) weak[n_weak + pos[0]] = w & CLF_TTF_INDEX_MASK;
+        if (tl) weak[n_weak + pos[0]] = make_uint4(t.t0, t.t1, t.t2, 0u);
         n_weak += tot[0];
     }
     __syncthreads();
@@ -947,7 +1089,7 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
         uint4 tl = make_uint4(0, 0, 0, 0);
         bool ok = false;
         if (i < n_weak) {
-            tl = tracklets[weak[i]];
+            tl = weak[i];
             ok = !used.get((int)tl.x) && !used.get((int)tl.y) && !used.get((int)tl.z);
         }
         int pos[3], tot[3];

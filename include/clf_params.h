@@ -49,7 +49,8 @@
 
 /* ---- Limits of this implementation ----------------------------------- */
 #define CLF_MAX_SENSORS                256      /* sensors per event          */
-#define CLF_MAX_SENSOR_HITS            65535    /* hits on one sensor         */
+#define CLF_MAX_SENSOR_HITS            32767    /* hits on one sensor (16-bit
+                                                   window indices)            */
 #define CLF_MAX_EVENT_HITS             (1 << 27)
 
 /* Output record, byte-identical to the reference's struct Track. */
