@@ -82,6 +82,58 @@ def cpu_baseline(blobs, gpu_outs, threads: int):
     }, identical
 
 
+def serving_runs(ctx, a):
+    """Serving benchmark: config-2-style events (the serve256 pool) arrive as
+    an open-loop Poisson stream at fractions of the batched peak measured on
+    the same kind of events; every result is checked against the plain path
+    by digest."""
+    import numpy as np
+
+    from cl_forward_amd import generator
+    from cl_forward_amd.search import MODE_BATCHED, MODE_PLAIN, Server, tracks_digest
+
+    pool = generator.workload_blobs("serve256")
+    peak_events = [pool[i % len(pool)] for i in range(1000)]
+    with ctx.batch(peak_events) as b:
+        t = b.time(MODE_BATCHED, warmup=2, reps=10)
+    peak = 1000.0 / (t["ms_run"] / 1e3)
+    outs, _ = ctx.search(pool, MODE_PLAIN)
+    want = np.array([tracks_digest(o) for o in outs], np.uint64)
+    conf = {"slots": a.serve_slots, "max_batch_events": a.serve_batch,
+            "max_batch_hits": a.serve_batch * 4096, "max_wait_us": a.serve_wait_us}
+    log(f"serving: batched peak on the pool {peak:.0f} events/s; server {conf}")
+    rows = []
+    with Server(ctx, **conf) as srv:
+        # warm-up replay (first-touch of pinned pages, thread start-up)
+        pick, arr = generator.serve_arrivals(0.1, 0.1 * peak, 2000, len(pool))
+        srv.replay(pool, pick, arr, submit_threads=a.serve_threads)
+        for frac in a.serve_rates:
+            pick, arr = generator.serve_arrivals(frac, frac * peak, a.serve_events, len(pool))
+            rec, summ = srv.replay(pool, pick, arr, submit_threads=a.serve_threads)
+            ok = rec["status"] == 0
+            same = int((rec["digest"][ok] == want[pick[ok]]).sum())
+            row = {"fraction_of_peak": frac,
+                   "offered_events_per_s": round(frac * peak, 1),
+                   "sustained_events_per_s": round(summ["events_per_s"], 1),
+                   "p50_ms": round(summ["p50_ns"] / 1e6, 4),
+                   "p90_ms": round(summ["p90_ns"] / 1e6, 4),
+                   "p99_ms": round(summ["p99_ns"] / 1e6, 4),
+                   "max_ms": round(summ["max_ns"] / 1e6, 4),
+                   "completed": summ["completed"], "rejected": summ["rejected"],
+                   "identical_to_plain": same,
+                   "mean_batch_events": round(summ["mean_batch_events"], 1),
+                   "mean_wait_ms": round(float(np.mean(rec["wait_ns"][ok])) / 1e6, 4) if ok.any() else None,
+                   "max_submit_lag_us": round(summ["max_submit_lag_ns"] / 1e3, 1)}
+            log(f"serving {frac:.2f}: {json.dumps(row)}")
+            rows.append(row)
+    return {"pool": f"serve256: {len(pool)} events, Poisson(150) tracks, seeds 1001..1256",
+            "arrivals": f"Poisson, {a.serve_events} per rate, seed 4000 + percent",
+            "latency": "scheduled arrival to tracks visible in host memory",
+            "peak_events_per_s": round(peak, 1), "server": conf,
+            "submit_threads": a.serve_threads, "host_cpus": len(os.sched_getaffinity(0)),
+            "rates": rows}
+
+
 def load_traffic(workload: str):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -109,6 +161,14 @@ def main(argv=None):
                     help="launch the kernels directly instead of replaying a graph")
     ap.add_argument("--probe-copy", type=int, default=0, metavar="BYTES",
                     help="only run the calibration copy kernel over BYTES and exit")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving runs")
+    ap.add_argument("--serve-only", action="store_true", help="only the serving runs")
+    ap.add_argument("--serve-events", type=int, default=100000)
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[0.25, 0.5, 0.75, 1.0])
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-batch", type=int, default=256)
+    ap.add_argument("--serve-wait-us", type=int, default=100)
+    ap.add_argument("--serve-threads", type=int, default=4)
     ap.add_argument("--profile-phases", action="store_true",
                     help="only print the per-phase time breakdown of one plain and one batched run")
     a = ap.parse_args(argv)
@@ -175,6 +235,12 @@ def main(argv=None):
         print(json.dumps(report), flush=True)
         return 0
 
+    if a.serve_only:
+        batch.close()
+        print(json.dumps({"workload": "serve256", "serving": serving_runs(ctx, a)}), flush=True)
+        ctx.close()
+        return 0
+
     # results of the batched path (and the plain path: identical tracks)
     batch.run(MODE_BATCHED)
     outs, stats = batch.results()
@@ -230,6 +296,11 @@ def main(argv=None):
         cpu, oracle_identical = cpu_baseline(blobs, outs, threads)
         log(f"cpu: {cpu['value']} events/s; oracle-identical events {oracle_identical}/{n_events}")
 
+    serving = None if a.no_serve else serving_runs(ctx, a)
+    at50 = None
+    if serving:
+        at50 = next((r for r in serving["rates"] if abs(r["fraction_of_peak"] - 0.5) < 1e-9), None)
+
     line = {
         "metric": "events/s on 1 MI355X, batched vs plain, identical tracks; p50/p99 latency",
         "value": round(value, 1),
@@ -261,6 +332,9 @@ def main(argv=None):
         "roofline": roofline,
         "cpu_baseline": cpu,
         "vs_cpu_baseline": round(value / cpu["value"], 1) if cpu else None,
+        "serving_p50_ms_at_50pct": at50["p50_ms"] if at50 else None,
+        "serving_p99_ms_at_50pct": at50["p99_ms"] if at50 else None,
+        "serving": serving,
     }
     batch.close()
     ctx.close()

@@ -52,6 +52,60 @@ class Timing(ctypes.Structure):
     _fields_ = [("ms_run", ctypes.c_float), ("ms_search", ctypes.c_float), ("reps", ctypes.c_int32)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("slots", ctypes.c_int32),
+        ("max_batch_events", ctypes.c_int32),
+        ("max_batch_hits", ctypes.c_int64),
+        ("max_wait_us", ctypes.c_int32),
+        ("max_requests", ctypes.c_int32),
+        ("arena_bytes", ctypes.c_int64),
+    ]
+
+
+class Served(ctypes.Structure):
+    _fields_ = [
+        ("ticket", ctypes.c_uint64),
+        ("tag", ctypes.c_uint64),
+        ("status", ctypes.c_int32),
+        ("batch_events", ctypes.c_int32),
+        ("stats", EventStats),
+        ("n_tracks", ctypes.c_int64),
+        ("tracks", ctypes.POINTER(ctypes.c_uint8)),
+        ("t_submit_ns", ctypes.c_int64),
+        ("t_launch_ns", ctypes.c_int64),
+        ("t_done_ns", ctypes.c_int64),
+    ]
+
+
+class ReplayRecord(ctypes.Structure):
+    _fields_ = [
+        ("latency_ns", ctypes.c_int64),
+        ("wait_ns", ctypes.c_int64),
+        ("digest", ctypes.c_uint64),
+        ("n_tracks", ctypes.c_int32),
+        ("status", ctypes.c_int32),
+        ("batch_events", ctypes.c_int32),
+        ("flags", ctypes.c_uint32),
+    ]
+
+
+class ReplaySummary(ctypes.Structure):
+    _fields_ = [
+        ("wall_s", ctypes.c_double),
+        ("events_per_s", ctypes.c_double),
+        ("completed", ctypes.c_int64),
+        ("rejected", ctypes.c_int64),
+        ("p50_ns", ctypes.c_int64),
+        ("p90_ns", ctypes.c_int64),
+        ("p99_ns", ctypes.c_int64),
+        ("max_ns", ctypes.c_int64),
+        ("batches", ctypes.c_int64),
+        ("mean_batch_events", ctypes.c_double),
+        ("max_submit_lag_ns", ctypes.c_double),
+    ]
+
+
 class ClfError(RuntimeError):
     """A C-ABI call returned a status other than CLF_OK."""
 
@@ -102,6 +156,18 @@ def load():
                                                ctypes.POINTER(ctypes.POINTER(ctypes.c_int32)),
                                                ctypes.POINTER(ctypes.POINTER(ctypes.c_int32)),
                                                ctypes.POINTER(ctypes.c_uint32)]),
+        "clf_server_default_config": (None, [ctypes.POINTER(ServerConfig)]),
+        "clf_server_create": (ctypes.c_int, [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]),
+        "clf_server_destroy": (None, [vp]),
+        "clf_server_submit": (ctypes.c_int, [vp, u8p, ctypes.c_size_t, ctypes.c_uint64,
+                                             ctypes.POINTER(ctypes.c_uint64)]),
+        "clf_server_wait": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(Served)]),
+        "clf_server_release": (ctypes.c_int, [vp, ctypes.c_uint64]),
+        "clf_server_replay": (ctypes.c_int, [vp, ctypes.c_int32, u8pp, sz_p, ctypes.c_int64,
+                                             ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64),
+                                             ctypes.c_int32, ctypes.POINTER(ReplayRecord),
+                                             ctypes.POINTER(ReplaySummary)]),
+        "clf_tracks_digest": (ctypes.c_uint64, [u8p, ctypes.c_int64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -117,7 +183,9 @@ EXPORTED = (
     "clf_context_destroy", "clf_context_set_config", "clf_context_get_config",
     "clf_context_num_cus", "clf_context_synchronize", "clf_event_track_capacity", "clf_batch_create", "clf_batch_destroy",
     "clf_batch_run", "clf_batch_results", "clf_batch_time", "clf_search", "clf_presort_event",
-    "clf_fill_candidates", "clf_probe_copy", "clf_batch_profile",
+    "clf_fill_candidates", "clf_probe_copy", "clf_batch_profile", "clf_server_default_config",
+    "clf_server_create", "clf_server_destroy", "clf_server_submit", "clf_server_wait",
+    "clf_server_release", "clf_server_replay", "clf_tracks_digest",
 )
 
 

@@ -21,7 +21,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "libclf.so")
-SOURCES = ["clf_kernels.hip", "clf_api.cpp", "clf_presort.cpp", "clf_shims.cpp"]
+SOURCES = ["clf_kernels.hip", "clf_api.cpp", "clf_serve.cpp", "clf_presort.cpp", "clf_shims.cpp"]
 
 
 def hipcc() -> str:
@@ -67,7 +67,7 @@ def build(arch: str = "gfx950", jobs: int = 4, verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as pool:
         objs = list(pool.map(compile_one, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [cc, f"--offload-arch={arch}", "-shared", "-fPIC", *objs, "-o", tmp]
+    cmd = [cc, f"--offload-arch={arch}", "-shared", "-fPIC", "-pthread", *objs, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -1,7 +1,5 @@
 // clf_api.cpp — host side of the C ABI (include/clf.h): contexts, resident
 // batches, launches, graphs, read-back and the host presort.
-#include <hip/hip_runtime.h>
-
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -9,24 +7,15 @@
 #include <string>
 #include <vector>
 
-#include "clf.h"
-#include "clf_device.h"
-
-namespace clf {
-hipError_t launch_search(const SearchArgs& a, int block, int lds_hits, int grid, hipStream_t stream);
-bool config_supported(int block, int lds_hits);
-hipError_t launch_zero_i32(int32_t* p, int n, hipStream_t stream);
-hipError_t launch_pack(const EventDesc* ev, const EventResult* res, int n, int64_t* prefix,
-                       const clf_track* tracks, uint32_t* packed, hipStream_t stream);
-hipError_t occupancy(int block, int lds_hits, int* blocks_per_cu);
-hipError_t launch_probe_copy(const void* src, void* dst, int64_t bytes, hipStream_t stream);
-}  // namespace clf
+#include "clf_host.h"
 
 using namespace clf;
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
+
+namespace clf {
 
 clf_status fail(clf_status st, const std::string& msg) {
     g_last_error = msg;
@@ -38,79 +27,12 @@ clf_status hip_fail(hipError_t e, const char* what) {
     return e == hipErrorOutOfMemory ? CLF_E_OUT_OF_MEMORY : CLF_E_DEVICE;
 }
 
-#define HIP_TRY(expr)                                        \
-    do {                                                     \
-        hipError_t _e = (expr);                              \
-        if (_e != hipSuccess) return hip_fail(_e, #expr);    \
-    } while (0)
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-    hipError_t ensure(size_t bytes) {
-        if (bytes == 0) bytes = 256;
-        if (bytes <= cap) return hipSuccess;
-        if (p) {
-            hipError_t e = hipFree(p);
-            p = nullptr;
-            cap = 0;
-            if (e != hipSuccess) return e;
-        }
-        hipError_t e = hipMalloc(&p, bytes);
-        if (e == hipSuccess) cap = bytes;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-    template <class T>
-    T* as() const {
-        return static_cast<T*>(p);
-    }
-};
-
-struct PinnedBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-    hipError_t ensure(size_t bytes) {
-        if (bytes == 0) bytes = 256;
-        if (bytes <= cap) return hipSuccess;
-        if (p) {
-            (void)hipHostFree(p);
-            p = nullptr;
-            cap = 0;
-        }
-        hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
-        if (e == hipSuccess) cap = bytes;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-    template <class T>
-    T* as() const {
-        return static_cast<T*>(p);
-    }
-};
-
-int32_t rd_i32(const uint8_t* p) {
-    int32_t v;
-    std::memcpy(&v, p, 4);
-    return v;
-}
-
-// Header checks of one blob; fills S and N.  Same rules as the oracle.
 bool validate_event(const uint8_t* b, size_t size, int32_t& S, int32_t& N) {
     if (!b || size < 8) return false;
     S = rd_i32(b);
     N = rd_i32(b + 4);
     if (S < 0 || S > CLF_MAX_SENSORS || N < 0 || N > CLF_MAX_EVENT_HITS) return false;
-    const size_t need = 8 + 12 * (size_t)S + 16 * (size_t)N;
-    if (size < need) return false;
+    if (size < event_bytes(S, N)) return false;
     for (int s = 0; s < S; ++s) {
         const int32_t st = rd_i32(b + 8 + 4 * ((size_t)S + s));
         const int32_t n = rd_i32(b + 8 + 4 * (2 * (size_t)S + s));
@@ -119,8 +41,6 @@ bool validate_event(const uint8_t* b, size_t size, int32_t& S, int32_t& N) {
     return true;
 }
 
-// Seeding and forwarding cost grows with hits on sensors two apart; used
-// only to order the work queue (largest first).
 double event_cost(const uint8_t* b, int32_t S, int32_t N) {
     double c = N;
     for (int s = 2; s < S; ++s) {
@@ -131,9 +51,93 @@ double event_cost(const uint8_t* b, int32_t S, int32_t N) {
     return c;
 }
 
-inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+hipError_t Resident::ensure(int32_t n, size_t blob_bytes, int64_t hits, int64_t cap) {
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t r) {
+        if (e == hipSuccess) e = r;
+    };
+    chk(blobs.ensure(blob_bytes));
+    chk(desc.ensure(sizeof(EventDesc) * (size_t)n));
+    chk(order.ensure(sizeof(int32_t) * (size_t)n));
+    chk(queue.ensure(sizeof(int32_t) * ((size_t)n + 1)));
+    chk(cand.ensure(sizeof(int4) * (size_t)hits));
+    chk(used.ensure((size_t)hits));
+    chk(tracks.ensure(sizeof(clf_track) * (size_t)cap));
+    chk(ttf_a.ensure(sizeof(TtfEntry) * (size_t)cap));
+    chk(ttf_b.ensure(sizeof(TtfEntry) * (size_t)cap));
+    chk(weak.ensure(sizeof(uint4) * (size_t)cap));
+    chk(results.ensure(sizeof(EventResult) * (size_t)n));
+    chk(prefix.ensure(sizeof(int64_t) * ((size_t)n + 1)));
+    chk(packed.ensure(sizeof(clf_track) * (size_t)cap));
+    return e;
+}
 
-}  // namespace
+void Resident::release() {
+    DevBuf* all[] = {&blobs, &desc,  &order, &queue,   &cand,   &used,  &tracks,
+                     &ttf_a, &ttf_b, &weak,  &results, &prefix, &packed};
+    for (DevBuf* d : all) d->release();
+}
+
+SearchArgs Resident::args(int32_t n) const {
+    SearchArgs a;
+    a.blobs = blobs.as<uint8_t>();
+    a.ev = desc.as<EventDesc>();
+    a.order = order.as<int32_t>();
+    a.first_item = 0;
+    a.n_items = n;
+    a.queue_head = queue.as<int32_t>();
+    a.grain = 1;
+    a.cand = cand.as<int4>();
+    a.used = used.as<uint8_t>();
+    a.tracks = tracks.as<clf_track>();
+    a.ttf_a = ttf_a.as<TtfEntry>();
+    a.ttf_b = ttf_b.as<TtfEntry>();
+    a.weak = weak.as<uint4>();
+    a.results = results.as<EventResult>();
+    a.fill_only = 0;
+    a.prof = nullptr;
+    return a;
+}
+
+int resolved_grid(const clf_launch_config& cfg, int n_cu, int n_items) {
+    int grid = cfg.grid_size;
+    if (grid <= 0) {
+        int per_cu = 0;
+        if (occupancy(cfg.block_size, cfg.lds_hits, &per_cu) != hipSuccess || per_cu <= 0)
+            per_cu = 1;
+        grid = n_cu * per_cu;
+    }
+    return std::max(1, std::min(grid, n_items));
+}
+
+clf_status enqueue_search(const Resident& r, int32_t n, clf_mode mode,
+                          const clf_launch_config& cfg, int n_cu, hipStream_t st, hipEvent_t t0,
+                          hipEvent_t t1) {
+    if (n == 0) return CLF_OK;
+    // queue counters: one per launch (n+1 so plain mode has one per event)
+    HIP_TRY(launch_zero_i32(r.queue.as<int32_t>(), n + 1, st));
+    SearchArgs a = r.args(n);
+    if (t0) HIP_TRY(hipEventRecord(t0, st));
+    if (mode == CLF_MODE_BATCHED) {
+        HIP_TRY(launch_search(a, cfg.block_size, cfg.lds_hits, resolved_grid(cfg, n_cu, n), st));
+    } else {
+        // plain: event i alone, in its own launch of one workgroup
+        a.order = nullptr;
+        a.n_items = 1;
+        for (int i = 0; i < n; ++i) {
+            a.first_item = i;
+            a.queue_head = r.queue.as<int32_t>() + 1 + i;
+            HIP_TRY(launch_search(a, cfg.block_size, cfg.lds_hits, 1, st));
+        }
+    }
+    if (t1) HIP_TRY(hipEventRecord(t1, st));
+    HIP_TRY(launch_pack(r.desc.as<EventDesc>(), r.results.as<EventResult>(), n,
+                        r.prefix.as<int64_t>(), r.tracks.as<clf_track>(),
+                        r.packed.as<uint32_t>(), st));
+    return CLF_OK;
+}
+
+}  // namespace clf
 
 struct clf_context {
     int device = 0;
@@ -148,8 +152,7 @@ struct clf_batch {
     int32_t n = 0;
     int64_t total_hits = 0, total_cap = 0;
     size_t blob_bytes = 0;
-    DevBuf blobs, desc, order, queue, cand, used, tracks, ttf_a, ttf_b, weak, results,
-        prefix, packed;
+    Resident dev;
     PinnedBuf h_blobs, h_results, h_prefix, h_packed;
     std::vector<EventDesc> h_desc;
     std::vector<int32_t> h_order;
@@ -159,68 +162,18 @@ struct clf_batch {
     bool results_valid = false;
 };
 
+namespace clf {
+int context_device(const clf_context* ctx) { return ctx->device; }
+}  // namespace clf
+
 namespace {
 
 hipError_t current_device(clf_context* ctx) { return hipSetDevice(ctx->device); }
 
-int resolved_grid(clf_context* ctx, int n_items) {
-    int grid = ctx->cfg.grid_size;
-    if (grid <= 0) {
-        int per_cu = 0;
-        if (occupancy(ctx->cfg.block_size, ctx->cfg.lds_hits, &per_cu) != hipSuccess || per_cu <= 0)
-            per_cu = 1;
-        grid = ctx->n_cu * per_cu;
-    }
-    return std::max(1, std::min(grid, n_items));
-}
-
-SearchArgs make_args(clf_batch* b) {
-    SearchArgs a;
-    a.blobs = b->blobs.as<uint8_t>();
-    a.ev = b->desc.as<EventDesc>();
-    a.order = b->order.as<int32_t>();
-    a.first_item = 0;
-    a.n_items = b->n;
-    a.queue_head = b->queue.as<int32_t>();
-    a.grain = 1;
-    a.cand = b->cand.as<int4>();
-    a.used = b->used.as<uint8_t>();
-    a.tracks = b->tracks.as<clf_track>();
-    a.ttf_a = b->ttf_a.as<TtfEntry>();
-    a.ttf_b = b->ttf_b.as<TtfEntry>();
-    a.weak = b->weak.as<uint4>();
-    a.results = b->results.as<EventResult>();
-    a.fill_only = 0;
-    a.prof = nullptr;
-    return a;
-}
-
 // The kernels of one run, enqueued on the context stream.
 clf_status enqueue_run(clf_context* ctx, clf_batch* b, clf_mode mode, bool time_search) {
-    hipStream_t st = ctx->stream;
-    const int n = b->n;
-    if (n == 0) return CLF_OK;
-    // queue counters: one per launch (n+1 so plain mode has one per event)
-    HIP_TRY(launch_zero_i32(b->queue.as<int32_t>(), n + 1, st));
-    SearchArgs a = make_args(b);
-    if (time_search) HIP_TRY(hipEventRecord(ctx->ev2, st));
-    if (mode == CLF_MODE_BATCHED) {
-        HIP_TRY(launch_search(a, ctx->cfg.block_size, ctx->cfg.lds_hits, resolved_grid(ctx, n), st));
-    } else {
-        // plain: event i alone, in its own launch of one workgroup
-        a.order = nullptr;
-        a.n_items = 1;
-        for (int i = 0; i < n; ++i) {
-            a.first_item = i;
-            a.queue_head = b->queue.as<int32_t>() + 1 + i;
-            HIP_TRY(launch_search(a, ctx->cfg.block_size, ctx->cfg.lds_hits, 1, st));
-        }
-    }
-    if (time_search) HIP_TRY(hipEventRecord(ctx->ev3, st));
-    HIP_TRY(launch_pack(b->desc.as<EventDesc>(), b->results.as<EventResult>(), n,
-                        b->prefix.as<int64_t>(), b->tracks.as<clf_track>(),
-                        b->packed.as<uint32_t>(), st));
-    return CLF_OK;
+    return enqueue_search(b->dev, b->n, mode, ctx->cfg, ctx->n_cu, ctx->stream,
+                          time_search ? ctx->ev2 : nullptr, time_search ? ctx->ev3 : nullptr);
 }
 
 bool same_cfg(const clf_launch_config& x, const clf_launch_config& y) {
@@ -358,10 +311,7 @@ void clf_batch_destroy(clf_batch* b) {
         (void)hipStreamSynchronize(b->ctx->stream);
     }
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
-    DevBuf* dev[] = {&b->blobs, &b->desc, &b->order, &b->queue, &b->cand, &b->used, &b->tracks,
-                     &b->ttf_a, &b->ttf_b, &b->weak, &b->results, &b->prefix,
-                     &b->packed};
-    for (DevBuf* d : dev) d->release();
+    b->dev.release();
     b->h_blobs.release();
     b->h_results.release();
     b->h_prefix.release();
@@ -416,19 +366,7 @@ clf_status clf_batch_create(clf_context* ctx, int32_t n, const uint8_t* const* e
         if (e == hipSuccess) e = r;
     };
     chk(b->h_blobs.ensure(off));
-    chk(b->blobs.ensure(off));
-    chk(b->desc.ensure(sizeof(EventDesc) * (size_t)n));
-    chk(b->order.ensure(sizeof(int32_t) * (size_t)n));
-    chk(b->queue.ensure(sizeof(int32_t) * ((size_t)n + 1)));
-    chk(b->cand.ensure(sizeof(int4) * (size_t)hits));
-    chk(b->used.ensure((size_t)hits));
-    chk(b->tracks.ensure(sizeof(clf_track) * (size_t)cap));
-    chk(b->ttf_a.ensure(sizeof(TtfEntry) * (size_t)cap));
-    chk(b->ttf_b.ensure(sizeof(TtfEntry) * (size_t)cap));
-    chk(b->weak.ensure(sizeof(uint4) * (size_t)cap));
-    chk(b->results.ensure(sizeof(EventResult) * (size_t)n));
-    chk(b->prefix.ensure(sizeof(int64_t) * ((size_t)n + 1)));
-    chk(b->packed.ensure(sizeof(clf_track) * (size_t)cap));
+    chk(b->dev.ensure(n, off, hits, cap));
     chk(b->h_results.ensure(sizeof(EventResult) * (size_t)n));
     chk(b->h_prefix.ensure(sizeof(int64_t) * ((size_t)n + 1)));
     if (e != hipSuccess) {
@@ -442,10 +380,10 @@ clf_status clf_batch_create(clf_context* ctx, int32_t n, const uint8_t* const* e
         std::memcpy(hb + d.blob_off, events[i], 8 + 12 * (size_t)d.n_sensors + 16 * (size_t)d.n_hits);
     }
     hipStream_t st = ctx->stream;
-    chk(hipMemcpyAsync(b->blobs.p, hb, off, hipMemcpyHostToDevice, st));
-    chk(hipMemcpyAsync(b->desc.p, b->h_desc.data(), sizeof(EventDesc) * (size_t)n,
+    chk(hipMemcpyAsync(b->dev.blobs.p, hb, off, hipMemcpyHostToDevice, st));
+    chk(hipMemcpyAsync(b->dev.desc.p, b->h_desc.data(), sizeof(EventDesc) * (size_t)n,
                        hipMemcpyHostToDevice, st));
-    chk(hipMemcpyAsync(b->order.p, b->h_order.data(), sizeof(int32_t) * (size_t)n,
+    chk(hipMemcpyAsync(b->dev.order.p, b->h_order.data(), sizeof(int32_t) * (size_t)n,
                        hipMemcpyHostToDevice, st));
     chk(hipStreamSynchronize(st));
     if (e != hipSuccess) {
@@ -469,16 +407,16 @@ clf_status clf_batch_results(clf_context* ctx, clf_batch* b, clf_result* out) {
     const int n = b->n;
     hipStream_t st = ctx->stream;
     if (n > 0) {
-        HIP_TRY(hipMemcpyAsync(b->h_results.p, b->results.p, sizeof(EventResult) * (size_t)n,
+        HIP_TRY(hipMemcpyAsync(b->h_results.p, b->dev.results.p, sizeof(EventResult) * (size_t)n,
                                hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(b->h_prefix.p, b->prefix.p, sizeof(int64_t) * ((size_t)n + 1),
+        HIP_TRY(hipMemcpyAsync(b->h_prefix.p, b->dev.prefix.p, sizeof(int64_t) * ((size_t)n + 1),
                                hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         const int64_t total = b->h_prefix.as<int64_t>()[n];
         if (total < 0 || total > b->total_cap) return fail(CLF_E_DEVICE, "inconsistent track count");
         HIP_TRY(b->h_packed.ensure(sizeof(clf_track) * (size_t)total));
         if (total > 0)
-            HIP_TRY(hipMemcpyAsync(b->h_packed.p, b->packed.p, sizeof(clf_track) * (size_t)total,
+            HIP_TRY(hipMemcpyAsync(b->h_packed.p, b->dev.packed.p, sizeof(clf_track) * (size_t)total,
                                    hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
     } else {
@@ -549,17 +487,17 @@ clf_status clf_batch_profile(clf_context* ctx, clf_batch* b, clf_mode mode, uint
         ~Guard() { p.release(); }
     } guard{prof};
     hipStream_t st = ctx->stream;
-    HIP_TRY(launch_zero_i32(b->queue.as<int32_t>(), n + 1, st));
-    SearchArgs a = make_args(b);
+    HIP_TRY(launch_zero_i32(b->dev.queue.as<int32_t>(), n + 1, st));
+    SearchArgs a = b->dev.args(n);
     a.prof = prof.as<unsigned long long>();
     if (mode == CLF_MODE_BATCHED) {
-        HIP_TRY(launch_search(a, ctx->cfg.block_size, ctx->cfg.lds_hits, resolved_grid(ctx, n), st));
+        HIP_TRY(launch_search(a, ctx->cfg.block_size, ctx->cfg.lds_hits, resolved_grid(ctx->cfg, ctx->n_cu, n), st));
     } else {
         a.order = nullptr;
         a.n_items = 1;
         for (int i = 0; i < n; ++i) {
             a.first_item = i;
-            a.queue_head = b->queue.as<int32_t>() + 1 + i;
+            a.queue_head = b->dev.queue.as<int32_t>() + 1 + i;
             HIP_TRY(launch_search(a, ctx->cfg.block_size, ctx->cfg.lds_hits, 1, st));
         }
     }
@@ -610,16 +548,16 @@ clf_status clf_fill_candidates(clf_context* ctx, int32_t n, const uint8_t* const
     } guard{b};
     if (n == 0) return CLF_OK;
     hipStream_t st = ctx->stream;
-    HIP_TRY(launch_zero_i32(b->queue.as<int32_t>(), n + 1, st));
-    SearchArgs a = make_args(b);
+    HIP_TRY(launch_zero_i32(b->dev.queue.as<int32_t>(), n + 1, st));
+    SearchArgs a = b->dev.args(n);
     a.fill_only = 1;
-    HIP_TRY(launch_search(a, ctx->cfg.block_size, ctx->cfg.lds_hits, resolved_grid(ctx, n), st));
+    HIP_TRY(launch_search(a, ctx->cfg.block_size, ctx->cfg.lds_hits, resolved_grid(ctx->cfg, ctx->n_cu, n), st));
     std::vector<int4> cand((size_t)std::max<int64_t>(b->total_hits, 1));
     std::vector<EventResult> res(n);
     if (b->total_hits > 0)
-        HIP_TRY(hipMemcpyAsync(cand.data(), b->cand.p, sizeof(int4) * (size_t)b->total_hits,
+        HIP_TRY(hipMemcpyAsync(cand.data(), b->dev.cand.p, sizeof(int4) * (size_t)b->total_hits,
                                hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(res.data(), b->results.p, sizeof(EventResult) * (size_t)n,
+    HIP_TRY(hipMemcpyAsync(res.data(), b->dev.results.p, sizeof(EventResult) * (size_t)n,
                            hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     for (int i = 0; i < n; ++i) {

@@ -1,0 +1,130 @@
+// clf_host.h — host-side pieces shared by the C ABI (clf_api.cpp) and the
+// serving loop (clf_serve.cpp): buffers, header checks, the resident device
+// buffers of one batch and the launch sequence of one search.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+
+#include "clf.h"
+#include "clf_device.h"
+
+namespace clf {
+
+// Kernel launchers (clf_kernels.hip).
+hipError_t launch_search(const SearchArgs& a, int block, int lds_hits, int grid, hipStream_t stream);
+bool config_supported(int block, int lds_hits);
+hipError_t launch_zero_i32(int32_t* p, int n, hipStream_t stream);
+hipError_t launch_pack(const EventDesc* ev, const EventResult* res, int n, int64_t* prefix,
+                       const clf_track* tracks, uint32_t* packed, hipStream_t stream);
+hipError_t occupancy(int block, int lds_hits, int* blocks_per_cu);
+hipError_t launch_probe_copy(const void* src, void* dst, int64_t bytes, hipStream_t stream);
+
+// Error reporting: the message is kept per thread for clf_last_error().
+clf_status fail(clf_status st, const std::string& msg);
+clf_status hip_fail(hipError_t e, const char* what);
+
+#define HIP_TRY(expr)                                               \
+    do {                                                            \
+        hipError_t _e = (expr);                                     \
+        if (_e != hipSuccess) return ::clf::hip_fail(_e, #expr);    \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes == 0) bytes = 256;
+        if (bytes <= cap) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            p = nullptr;
+            cap = 0;
+            if (e != hipSuccess) return e;
+        }
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipSuccess) cap = bytes;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes == 0) bytes = 256;
+        if (bytes <= cap) return hipSuccess;
+        if (p) {
+            (void)hipHostFree(p);
+            p = nullptr;
+            cap = 0;
+        }
+        hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+        if (e == hipSuccess) cap = bytes;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+inline int32_t rd_i32(const uint8_t* p) {
+    int32_t v;
+    std::memcpy(&v, p, 4);
+    return v;
+}
+
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Bytes of a well-formed event blob with S sensors and N hits.
+inline size_t event_bytes(int32_t S, int32_t N) { return 8 + 12 * (size_t)S + 16 * (size_t)N; }
+
+// Header checks of one blob; fills S and N.  Same rules as the oracle.
+bool validate_event(const uint8_t* b, size_t size, int32_t& S, int32_t& N);
+
+// Estimated relative cost of an event (hits on sensors two apart), used
+// only to order a work queue largest first.
+double event_cost(const uint8_t* b, int32_t S, int32_t N);
+
+// Device buffers of one batch of events.  Per-hit and per-track scratch is
+// sliced per event by EventDesc::hit_off / track_off.
+struct Resident {
+    DevBuf blobs, desc, order, queue, cand, used, tracks, ttf_a, ttf_b, weak, results, prefix,
+        packed;
+    // n events, blob_bytes of packed blobs, hits in total, track capacity
+    hipError_t ensure(int32_t n, size_t blob_bytes, int64_t hits, int64_t cap);
+    void release();
+    SearchArgs args(int32_t n) const;
+};
+
+// HIP device ordinal of a context.
+int context_device(const clf_context* ctx);
+
+// Persistent grid for n queued events under a launch configuration.
+int resolved_grid(const clf_launch_config& cfg, int n_cu, int n_items);
+
+// Enqueues one search of n resident events on `stream`: queue reset, the
+// search kernel(s), track prefix and packing.  t0/t1 (nullable) are
+// recorded right around the search kernel(s).
+clf_status enqueue_search(const Resident& r, int32_t n, clf_mode mode,
+                          const clf_launch_config& cfg, int n_cu, hipStream_t stream,
+                          hipEvent_t t0, hipEvent_t t1);
+
+}  // namespace clf

@@ -18,6 +18,10 @@ Workloads (seeds are part of the definition):
   1 % noise, event i seeded 2000*10**4 + i
 * ``stress1000`` 1,000 events, Poisson(15) vertices x Poisson(25) tracks,
   5 % noise, event i seeded 3000*10**4 + i
+* ``serve256`` the serving pool: 256 events drawn like ``plain51``
+  (Poisson(150) tracks), seeds 1001..1256; a serving run at fraction r of the
+  batched peak draws arrival gaps ~ Exp and pool indices from the generator
+  seeded 4000 + round(100 r) (:func:`serve_arrivals`)
 """
 from __future__ import annotations
 
@@ -157,7 +161,8 @@ def single_event(return_truth=False):
 
 
 def plain_event(i: int, return_truth=False):
-    """Event i (0..50) of the 51-event plain workload, seed 1001 + i."""
+    """Event i of the plain workload (0..50) and of the serving pool (0..255),
+    seed 1001 + i."""
     return generate_event(1001 + i, tracks_mean=150, noise_fraction=0.01, return_truth=return_truth)
 
 
@@ -173,11 +178,23 @@ def stress_event(i: int, return_truth=False):
                           noise_fraction=0.05, return_truth=return_truth)
 
 
+def serve_arrivals(fraction: float, rate_per_s: float, n: int, pool_size: int):
+    """Open-loop Poisson arrivals for the serving benchmark: (pick, arrival_ns),
+    the first arrival at 0."""
+    rng = np.random.default_rng(4000 + int(round(100 * fraction)))
+    gaps = rng.exponential(1e9 / rate_per_s, n)
+    arrival = np.cumsum(gaps)
+    arrival -= arrival[0]
+    pick = rng.integers(0, pool_size, n).astype(np.int32)
+    return pick, arrival.astype(np.int64)
+
+
 WORKLOADS = {
     "single": (1, lambda i: single_event()),
     "plain51": (51, plain_event),
     "batch1000": (1000, batch_event),
     "stress1000": (1000, stress_event),
+    "serve256": (256, plain_event),
 }
 
 

@@ -22,13 +22,14 @@ import ctypes
 import threading
 
 from . import _lib
-from ._lib import MODE_BATCHED, MODE_PLAIN, ClfError, EventStats, LaunchConfig, Result, Timing
+from ._lib import (MODE_BATCHED, MODE_PLAIN, ClfError, EventStats, LaunchConfig, ReplayRecord,
+                   ReplaySummary, Result, Served, ServerConfig, Timing)
 
 TRACK_BYTES = 100
 MAX_EVENTS_PER_LAUNCH = 16000  # GpuPixelSearchByTriplet.cpp:54
 
 __all__ = [
-    "Context", "Batch", "ClfError", "MODE_PLAIN", "MODE_BATCHED", "gpu_pixel_search_by_triplet",
+    "Context", "Batch", "Server", "ClfError", "tracks_digest", "MODE_PLAIN", "MODE_BATCHED", "gpu_pixel_search_by_triplet",
     "gpu_pixel_search_by_triplet_invocation", "independent_execute", "independent_post_execute",
     "invoke_parallel_search", "presort_event", "track_capacity",
 ]
@@ -217,6 +218,106 @@ class Batch:
         _lib.check(self._L.clf_batch_time(self.ctx.handle, self._h, int(mode), int(warmup),
                                           int(reps), ctypes.byref(t)), "clf_batch_time")
         return {"ms_run": float(t.ms_run), "ms_search": float(t.ms_search), "reps": int(t.reps)}
+
+
+def tracks_digest(tracks: bytes) -> int:
+    """clf_tracks_digest of packed Track records (len a multiple of 100)."""
+    L = _lib.load()
+    n = len(tracks) // TRACK_BYTES
+    if n == 0:
+        return int(L.clf_tracks_digest(None, 0))
+    buf = (ctypes.c_uint8 * len(tracks)).from_buffer_copy(tracks)
+    return int(L.clf_tracks_digest(buf, n))
+
+
+class Server:
+    """Serving loop (``clf_server``): submit events one at a time, get each
+    event's tracks when its batch finishes.  ``config`` keys are the fields
+    of clf_server_config; missing ones take the library defaults."""
+
+    def __init__(self, ctx: Context, **config):
+        self.ctx = ctx
+        self._L = ctx._L
+        c = ServerConfig()
+        self._L.clf_server_default_config(ctypes.byref(c))
+        for k, v in config.items():
+            if k not in dict(ServerConfig._fields_):
+                raise TypeError(f"unknown server option {k!r}")
+            setattr(c, k, int(v))
+        h = ctypes.c_void_p()
+        _lib.check(self._L.clf_server_create(ctx.handle, ctypes.byref(c), ctypes.byref(h)),
+                   "clf_server_create")
+        self._h = h
+        self.config = {k: int(getattr(c, k)) for k, _ in ServerConfig._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.clf_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, event: bytes, tag: int = 0) -> int:
+        b = bytes(event)
+        buf = (ctypes.c_uint8 * max(len(b), 1)).from_buffer_copy(b or b"\0")
+        t = ctypes.c_uint64()
+        _lib.check(self._L.clf_server_submit(self._h, buf, len(b), int(tag), ctypes.byref(t)),
+                   "clf_server_submit")
+        return int(t.value)
+
+    def wait(self, ticket: int, timeout_ms: int = -1, release: bool = True):
+        """(tracks bytes, stats dict, timing dict) of a request; releases it
+        unless release=False."""
+        o = Served()
+        _lib.check(self._L.clf_server_wait(self._h, int(ticket), int(timeout_ms), ctypes.byref(o)),
+                   "clf_server_wait")
+        try:
+            _lib.check(o.status, "served request")
+            raw = ctypes.string_at(o.tracks, o.n_tracks * TRACK_BYTES) if o.n_tracks else b""
+            stats = {k: int(getattr(o.stats, k)) for k, _ in EventStats._fields_ if k != "reserved"}
+            timing = {"submit_ns": int(o.t_submit_ns), "launch_ns": int(o.t_launch_ns),
+                      "done_ns": int(o.t_done_ns), "batch_events": int(o.batch_events)}
+        finally:
+            if release:
+                self.release(ticket)
+        return raw, stats, timing
+
+    def release(self, ticket: int):
+        _lib.check(self._L.clf_server_release(self._h, int(ticket)), "clf_server_release")
+
+    def replay(self, pool, pick, arrival_ns, submit_threads: int = 4):
+        """Open-loop load replay (clf_server_replay).  Returns (records as a
+        numpy structured array, summary dict)."""
+        import numpy as np
+
+        holders, ptrs, sizes = _event_arrays(pool)
+        pick = np.ascontiguousarray(pick, np.int32)
+        arr = np.ascontiguousarray(arrival_ns, np.int64)
+        n = len(pick)
+        recs = (ReplayRecord * max(n, 1))()
+        summ = ReplaySummary()
+        _lib.check(self._L.clf_server_replay(
+            self._h, len(pool), ptrs, sizes, n, pick.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+            arr.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), int(submit_threads), recs,
+            ctypes.byref(summ)), "clf_server_replay")
+        del holders
+        dt = np.dtype([(k, {ctypes.c_int64: "<i8", ctypes.c_uint64: "<u8", ctypes.c_int32: "<i4",
+                             ctypes.c_uint32: "<u4"}[t]) for k, t in ReplayRecord._fields_])
+        assert dt.itemsize == ctypes.sizeof(ReplayRecord)
+        rec = np.frombuffer(bytes(recs), dt, count=n).copy() if n else np.zeros(0, dt)
+        summary = {k: (float(getattr(summ, k)) if t is ctypes.c_double else int(getattr(summ, k)))
+                   for k, t in ReplaySummary._fields_}
+        return rec, summary
 
 
 # ---------------------------------------------------------------------------

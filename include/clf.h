@@ -157,6 +157,104 @@ clf_status clf_batch_profile(clf_context* ctx, clf_batch* batch, clf_mode mode, 
  * (bytes read + bytes written) calibrates HBM counters. */
 clf_status clf_probe_copy(clf_context* ctx, int64_t bytes, int32_t reps, float* ms_per_copy);
 
+/* ---- Serving ----------------------------------------------------------
+ * A server accepts events one at a time and returns each event's tracks
+ * as soon as the batch it joined has finished.  The reference has no
+ * serving loop: its Gaudi offload handler calls gpuPixelSearchByTriplet
+ * (GpuPixelSearchByTriplet.cpp:29-37) once per request with a blocking
+ * upload, launch and read-back (KernelInvoker.cpp:114-206).  Here a
+ * dispatcher thread keeps `slots` batches in flight, each on its own stream
+ * with resident device buffers, and forms the next batch from everything
+ * that arrived while the others ran (continuous batching).  Results are
+ * bit-identical to clf_search on the same event.
+ *
+ * Threading: clf_server_submit, _wait and _release may be called from any
+ * number of threads.  Admission: a submit that would exceed the pinned
+ * input arena, the request table or a slot's capacity returns
+ * CLF_E_CAPACITY at once (nothing is queued). */
+typedef struct clf_server clf_server;
+
+typedef struct clf_server_config {
+    int32_t slots;            /* batches in flight (streams), 1..8              */
+    int32_t max_batch_events; /* events per batch                               */
+    int64_t max_batch_hits;   /* hits per batch; sizes each slot's scratch      */
+    int32_t max_wait_us;      /* while another batch runs, a batch waits up to
+                                 this long to fill; an idle device launches at
+                                 once                                           */
+    int32_t max_requests;     /* submitted and not yet released                 */
+    int64_t arena_bytes;      /* pinned input arena                             */
+} clf_server_config;
+
+/* A finished request.  `tracks` points into the server's pinned output and
+ * stays valid until clf_server_release(ticket). */
+typedef struct clf_served {
+    uint64_t ticket;
+    uint64_t tag;             /* the submitter's tag                            */
+    int32_t status;           /* CLF_OK, or the error that ended the request    */
+    int32_t batch_events;     /* size of the batch the event ran in             */
+    clf_event_stats stats;
+    int64_t n_tracks;
+    const uint8_t* tracks;    /* n_tracks * 100 bytes                           */
+    int64_t t_submit_ns;      /* steady clock: submit returned                  */
+    int64_t t_launch_ns;      /* its batch was enqueued                         */
+    int64_t t_done_ns;        /* its tracks were visible on the host            */
+} clf_served;
+
+/* Fills *cfg with the defaults (4 slots, 256 events / 2^20 hits per batch,
+ * 100 us, 65536 requests, 512 MiB arena). */
+void clf_server_default_config(clf_server_config* cfg);
+/* Starts a server on the context's device with the context's launch
+ * configuration.  Allocates every buffer up front. */
+clf_status clf_server_create(clf_context* ctx, const clf_server_config* cfg, clf_server** out);
+/* Stops the dispatcher after the requests in flight have finished and frees
+ * everything; outstanding tracks pointers become invalid. */
+void clf_server_destroy(clf_server* server);
+/* Copies one event into the input arena and queues it.  A malformed blob is
+ * rejected with CLF_E_INVALID_ARGUMENT. */
+clf_status clf_server_submit(clf_server* server, const uint8_t* event, size_t size, uint64_t tag,
+                             uint64_t* ticket);
+/* Waits up to timeout_ms (< 0: no limit) for a request; CLF_E_CAPACITY on
+ * time-out (the request stays queued). */
+clf_status clf_server_wait(clf_server* server, uint64_t ticket, int32_t timeout_ms,
+                           clf_served* out);
+/* Gives a finished request's output back to the server. */
+clf_status clf_server_release(clf_server* server, uint64_t ticket);
+
+/* Load replay: arrival i submits pool event pick[i] at arrival_ns[i] after
+ * the start (open loop, from submit_threads threads), every result is
+ * digested (clf_tracks_digest) and released as it arrives.  Latency is
+ * counted from the scheduled arrival to the result visible on the host. */
+typedef struct clf_replay_record {
+    int64_t latency_ns;       /* -1 if rejected                                 */
+    int64_t wait_ns;          /* scheduled arrival to batch launch              */
+    uint64_t digest;
+    int32_t n_tracks;
+    int32_t status;           /* CLF_OK, or why the submit or the run failed    */
+    int32_t batch_events;
+    uint32_t flags;
+} clf_replay_record;
+
+typedef struct clf_replay_summary {
+    double wall_s;            /* first arrival to last result                   */
+    double events_per_s;      /* completed / wall_s                             */
+    int64_t completed, rejected;
+    int64_t p50_ns, p90_ns, p99_ns, max_ns;
+    int64_t batches;
+    double mean_batch_events;
+    double max_submit_lag_ns; /* how late the submitters ran at worst           */
+} clf_replay_summary;
+
+clf_status clf_server_replay(clf_server* server, int32_t n_pool, const uint8_t* const* pool,
+                             const size_t* sizes, int64_t n_arrivals, const int32_t* pick,
+                             const int64_t* arrival_ns, int32_t submit_threads,
+                             clf_replay_record* records, clf_replay_summary* summary);
+
+/* 64-bit digest of n_tracks packed Track records: eight multiply-xorshift
+ * lanes over the 64-bit words of each 64-byte block, the remaining 32-bit
+ * words spread over the lanes in turn, then a final mix with the count.
+ * Used to compare outputs of different paths without moving them. */
+uint64_t clf_tracks_digest(const uint8_t* tracks, int64_t n_tracks);
+
 /* x-presort of one event in place, per sensor, with the reference's
  * partition sequence so that tied x keep its order (preorder_by_x,
  * Tools.cpp:46-66, 117-164).  Host code; 0 or -1 for a malformed blob. */

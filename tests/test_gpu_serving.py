@@ -1,0 +1,117 @@
+"""Serving loop on the GPU: every served event's tracks must be byte-identical
+to the plain path's for the same event, whether it ran alone or inside a
+batch with others, and to the oracle's."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from cl_forward_amd.search import MODE_PLAIN, ClfError, Server, tracks_digest
+from tests.helpers import line_event
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def plain_ref(gpu_ctx, plain51_blobs):
+    outs, stats = gpu_ctx.search(plain51_blobs, MODE_PLAIN)
+    return outs, stats
+
+
+def test_serve_single_matches_plain_and_oracle(gpu_ctx, oracle_mod, single_blob):
+    ref, _ = gpu_ctx.search([single_blob], MODE_PLAIN)
+    want = oracle_mod.search_event(single_blob)[0]
+    with Server(gpu_ctx, slots=2, max_batch_events=16, max_batch_hits=1 << 16) as srv:
+        t = srv.submit(single_blob, tag=7)
+        raw, stats, timing = srv.wait(t, timeout_ms=20000)
+    assert raw == ref[0] == want
+    assert stats["n_tracks"] == len(raw) // 100
+    assert timing["batch_events"] == 1
+    assert timing["submit_ns"] <= timing["launch_ns"] <= timing["done_ns"]
+
+
+def test_serve_mix_equals_alone(gpu_ctx, plain51_blobs, plain_ref, mcdata_sorted):
+    """All events submitted at once (they share batches) and then one at a
+    time (each alone) give the plain path's bytes."""
+    ref_outs, _ = plain_ref
+    mc = [mcdata_sorted[k] for k in list(mcdata_sorted)[:12]]
+    mc_ref, _ = gpu_ctx.search(mc, MODE_PLAIN)
+    events = plain51_blobs + mc
+    want = ref_outs + mc_ref
+    with Server(gpu_ctx, slots=3, max_batch_events=32, max_batch_hits=1 << 18,
+                max_wait_us=200) as srv:
+        tickets = [srv.submit(e, tag=i) for i, e in enumerate(events)]
+        got = [srv.wait(t, timeout_ms=30000) for t in tickets]
+        assert any(g[2]["batch_events"] > 1 for g in got), "nothing was batched"
+        for i, (raw, stats, _) in enumerate(got):
+            assert raw == want[i], f"event {i} differs in the mix"
+        for i in (0, 5, 51, len(events) - 1):
+            raw, _, timing = srv.wait(srv.submit(events[i]), timeout_ms=30000)
+            assert timing["batch_events"] == 1
+            assert raw == want[i], f"event {i} differs alone"
+
+
+def test_serve_replay_digests(gpu_ctx, plain51_blobs, plain_ref):
+    """Open-loop replay of 4000 arrivals drawn from the 51 events; every
+    result's digest equals the plain path's."""
+    ref_outs, ref_stats = plain_ref
+    want = np.array([tracks_digest(o) for o in ref_outs], np.uint64)
+    rng = np.random.default_rng(4050)
+    n = 4000
+    pick = rng.integers(0, len(plain51_blobs), n).astype(np.int32)
+    arrival = np.cumsum(rng.exponential(1e9 / 100000.0, n)).astype(np.int64)
+    with Server(gpu_ctx, slots=4, max_batch_events=128, max_batch_hits=1 << 19) as srv:
+        rec, summ = srv.replay(plain51_blobs, pick, arrival, submit_threads=2)
+    ok = rec["status"] == 0
+    assert summ["rejected"] == int((~ok).sum()) == 0
+    assert summ["completed"] == n
+    assert np.array_equal(rec["digest"], want[pick])
+    assert np.array_equal(rec["n_tracks"], np.array([s["n_tracks"] for s in ref_stats])[pick])
+    assert (rec["latency_ns"] > 0).all() and summ["p99_ns"] >= summ["p50_ns"] > 0
+    assert summ["batches"] >= 1 and summ["mean_batch_events"] >= 1.0
+
+
+def test_serve_small_arena_wraps(gpu_ctx, plain51_blobs, plain_ref):
+    """A 1 MiB input arena holds about 25 of these events: the ring wraps
+    many times, submits beyond it are refused with CLF_E_CAPACITY, and every
+    accepted event is still exact."""
+    ref_outs, _ = plain_ref
+    want = np.array([tracks_digest(o) for o in ref_outs], np.uint64)
+    with Server(gpu_ctx, slots=2, max_batch_events=8, max_batch_hits=1 << 16,
+                arena_bytes=1 << 20) as srv:
+        for rnd in range(3):
+            for i, e in enumerate(plain51_blobs):
+                raw, _, _ = srv.wait(srv.submit(e), timeout_ms=30000)
+                assert raw == ref_outs[i], (rnd, i)
+        rng = np.random.default_rng(5)
+        n = 1500
+        pick = rng.integers(0, len(plain51_blobs), n).astype(np.int32)
+        arrival = np.cumsum(rng.exponential(1e9 / 200000.0, n)).astype(np.int64)
+        rec, summ = srv.replay(plain51_blobs, pick, arrival, submit_threads=2)
+    ok = rec["status"] == 0
+    assert summ["completed"] == int(ok.sum()) and summ["completed"] + summ["rejected"] == n
+    assert (rec["status"][~ok] == 5).all()
+    assert np.array_equal(rec["digest"][ok], want[pick[ok]])
+
+
+def test_serve_admission_errors(gpu_ctx, single_blob):
+    with Server(gpu_ctx, slots=1, max_batch_events=4, max_batch_hits=1000, max_requests=2) as srv:
+        with pytest.raises(ClfError) as e:
+            srv.submit(single_blob[:100])
+        assert e.value.status == 1
+        with pytest.raises(ClfError) as e:  # 2,500 hits > max_batch_hits
+            srv.submit(single_blob)
+        assert e.value.status == 5
+        small = line_event([(1.0, 2.0, 0.01, -0.02), (-3.0, 1.0, -0.03, 0.01)],
+                           [-300 + 22 * k for k in range(20)]).pack()
+        t1 = srv.submit(small)
+        t2 = srv.submit(small)
+        with pytest.raises(ClfError) as e:  # request table full until released
+            srv.submit(small)
+        assert e.value.status == 5
+        a = srv.wait(t1, timeout_ms=20000)[0]
+        b = srv.wait(t2, timeout_ms=20000)[0]
+        assert a == b
+        srv.wait(srv.submit(small), timeout_ms=20000)
+        with pytest.raises(ClfError):
+            srv.release(t1)  # already released

@@ -180,3 +180,99 @@ def test_bench_rejects_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_c_struct_layouts_match_ctypes(tmp_path):
+    """Every struct of clf.h has the same size and field offsets in C as in
+    the ctypes binding."""
+    import ctypes
+
+    from cl_forward_amd import _lib
+
+    structs = {"clf_event_stats": _lib.EventStats, "clf_launch_config": _lib.LaunchConfig,
+               "clf_result": _lib.Result, "clf_timing": _lib.Timing,
+               "clf_server_config": _lib.ServerConfig, "clf_served": _lib.Served,
+               "clf_replay_record": _lib.ReplayRecord, "clf_replay_summary": _lib.ReplaySummary}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "clf.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", f"-I{INCLUDE}", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
+def test_tracks_digest_properties(lib):
+    """The digest is a function of the bytes and the count: equal inputs
+    agree, a one-bit change or a reordering of records changes it."""
+    from cl_forward_amd.search import tracks_digest
+
+    rng = np.random.default_rng(3)
+    a = rng.integers(0, 2**32, 25 * 37, dtype=np.uint64).astype(np.uint32).tobytes()
+    assert tracks_digest(a) == tracks_digest(bytes(a))
+    b = bytearray(a)
+    b[1234] ^= 1
+    assert tracks_digest(bytes(b)) != tracks_digest(a)
+    swapped = a[100:200] + a[:100] + a[200:]
+    assert tracks_digest(swapped) != tracks_digest(a)
+    assert tracks_digest(a[:100]) != tracks_digest(a[:200])
+    assert tracks_digest(b"") == tracks_digest(b"")
+
+
+def test_serve_arrivals_reproducible():
+    from cl_forward_amd import generator
+
+    p1, a1 = generator.serve_arrivals(0.5, 1e5, 1000, 256)
+    p2, a2 = generator.serve_arrivals(0.5, 1e5, 1000, 256)
+    assert np.array_equal(p1, p2) and np.array_equal(a1, a2)
+    assert a1[0] == 0 and (np.diff(a1) >= 0).all()
+    assert 0 <= p1.min() and p1.max() < 256
+    mean_gap = float(np.diff(a1).mean())
+    assert 0.8e4 < mean_gap < 1.2e4  # 1e5 events/s -> 10 us
+
+
+def _digest_py(data: bytes, n_tracks: int) -> int:
+    """Independent restatement of clf_tracks_digest in Python integers."""
+    M = (1 << 64) - 1
+    P = 0x9E3779B97F4A7C15
+    h = [(0x243F6A8885A308D3 * (2 * l + 1)) & M for l in range(8)]
+
+    def step(x, w):
+        x = ((x ^ w) * P) & M
+        return x ^ (x >> 29)
+
+    nbytes = 100 * n_tracks
+    i = 0
+    while i + 64 <= nbytes:
+        for l in range(8):
+            h[l] = step(h[l], int.from_bytes(data[i + 8 * l: i + 8 * l + 8], "little"))
+        i += 64
+    l = 0
+    while i < nbytes:
+        h[l] = step(h[l], int.from_bytes(data[i: i + 4], "little"))
+        i += 4
+        l = (l + 1) & 7
+    r = (n_tracks * P) & M
+    for l in range(8):
+        r = ((r ^ h[l]) * P) & M
+        r ^= r >> 32
+    return r
+
+
+def test_tracks_digest_matches_restatement(lib):
+    """Every tail length (0..15 trailing words after the 64-byte blocks)."""
+    from cl_forward_amd.search import tracks_digest
+
+    rng = np.random.default_rng(11)
+    for n in list(range(0, 18)) + [37, 100]:
+        data = rng.integers(0, 256, 100 * n, dtype=np.uint8).tobytes()
+        assert tracks_digest(data) == _digest_py(data, n), n
