@@ -99,8 +99,11 @@ def serving_runs(ctx, a):
     peak = 1000.0 / (t["ms_run"] / 1e3)
     outs, _ = ctx.search(pool, MODE_PLAIN)
     want = np.array([tracks_digest(o) for o in outs], np.uint64)
-    conf = {"slots": a.serve_slots, "max_batch_events": a.serve_batch,
-            "max_batch_hits": a.serve_batch * 4096, "max_wait_us": a.serve_wait_us}
+    if a.serve_engine == "resident":
+        conf = {"engine": 0, "idle_us": 5000}
+    else:
+        conf = {"engine": 1, "slots": a.serve_slots, "max_batch_events": a.serve_batch,
+                "max_batch_hits": a.serve_batch * 4096, "max_wait_us": a.serve_wait_us}
     log(f"serving: batched peak on the pool {peak:.0f} events/s; server {conf}")
     rows = []
     with Server(ctx, **conf) as srv:
@@ -123,7 +126,11 @@ def serving_runs(ctx, a):
                    "identical_to_plain": same,
                    "mean_batch_events": round(summ["mean_batch_events"], 1),
                    "mean_wait_ms": round(float(np.mean(rec["wait_ns"][ok])) / 1e6, 4) if ok.any() else None,
-                   "max_submit_lag_us": round(summ["max_submit_lag_ns"] / 1e3, 1)}
+                   "max_submit_lag_us": round(summ["max_submit_lag_ns"] / 1e3, 1),
+                   "dev_us": {k: round(summ[f"dev_{k}_us"], 1) for k in ("copy_in", "search", "copy_out")},
+                   "dev_busy_frac": round(summ["dev_busy_frac"], 3),
+                   "host_submit_us": round(summ["host_submit_us"], 2),
+                   "host_digest_us": round(summ["host_digest_us"], 2)}
             log(f"serving {frac:.2f}: {json.dumps(row)}")
             rows.append(row)
     return {"pool": f"serve256: {len(pool)} events, Poisson(150) tracks, seeds 1001..1256",
@@ -165,6 +172,7 @@ def main(argv=None):
     ap.add_argument("--serve-only", action="store_true", help="only the serving runs")
     ap.add_argument("--serve-events", type=int, default=100000)
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[0.25, 0.5, 0.75, 1.0])
+    ap.add_argument("--serve-engine", choices=["resident", "batches"], default="resident")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-batch", type=int, default=256)
     ap.add_argument("--serve-wait-us", type=int, default=100)

@@ -54,13 +54,24 @@ class Timing(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [
+        ("engine", ctypes.c_int32),
+        ("max_requests", ctypes.c_int32),
+        ("arena_bytes", ctypes.c_int64),
+        ("out_arena_bytes", ctypes.c_int64),
+        ("max_event_hits", ctypes.c_int32),
+        ("idle_us", ctypes.c_int32),
         ("slots", ctypes.c_int32),
         ("max_batch_events", ctypes.c_int32),
         ("max_batch_hits", ctypes.c_int64),
         ("max_wait_us", ctypes.c_int32),
-        ("max_requests", ctypes.c_int32),
-        ("arena_bytes", ctypes.c_int64),
+        ("out_tracks_per_1k_hits", ctypes.c_int32),
+        ("out_min_tracks", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
+
+
+SERVE_RESIDENT = 0
+SERVE_BATCHES = 1
 
 
 class Served(ctypes.Structure):
@@ -103,6 +114,12 @@ class ReplaySummary(ctypes.Structure):
         ("batches", ctypes.c_int64),
         ("mean_batch_events", ctypes.c_double),
         ("max_submit_lag_ns", ctypes.c_double),
+        ("dev_copy_in_us", ctypes.c_double),
+        ("dev_search_us", ctypes.c_double),
+        ("dev_copy_out_us", ctypes.c_double),
+        ("dev_busy_frac", ctypes.c_double),
+        ("host_submit_us", ctypes.c_double),
+        ("host_digest_us", ctypes.c_double),
     ]
 
 

@@ -78,4 +78,67 @@ struct SearchArgs {
 
 constexpr int kProfSlots = 16;
 
+// ---- Resident serving kernel --------------------------------------------
+// A request published by the host in pinned memory.  `seq` is written last
+// (release): descriptor d of the ring is valid once seq == d + 1.
+struct alignas(16) ServeDesc {
+    uint64_t in_off;   // event blob in the input arena (256-byte aligned)
+    uint64_t out_off;  // byte offset of the request's output region (256-aligned)
+    int32_t n_hits;
+    int32_t n_sensors;
+    int32_t req;       // host request index, echoed in the completion
+    int32_t out_cap;   // Track records the output region holds
+    uint32_t seq;
+    uint32_t pad[3];
+};
+static_assert(sizeof(ServeDesc) == 48, "the kernel reads a descriptor as 16-byte words");
+
+// A finished request, appended by the device to a ring in pinned memory;
+// valid once seq == position + 1.
+enum CompStatus { COMP_OK = 0, COMP_SPILL = 1 };  // SPILL: more tracks than out_cap, none written
+
+struct CompRec {
+    uint32_t seq;
+    int32_t req;
+    int32_t wg;        // workgroup that ran it
+    int32_t status;    // CompStatus
+    EventResult res;
+    // device wall clock: claimed, event copied in, searched, tracks written
+    uint64_t t_claim, t_copied, t_searched, t_written;
+};
+
+// Control words in device memory.  HEAD (descriptors handed to workers),
+// TAIL (descriptors seen published) and DONE (completions written) persist
+// across launches; QUIT, IDLE_TAIL and the feeder's heartbeat BEAT (64-bit,
+// words 6-7) are cleared before each launch.
+enum ServeCtl {
+    CTL_HEAD = 0,
+    CTL_TAIL = 1,
+    CTL_QUIT = 2,
+    CTL_DONE = 3,
+    CTL_IDLE_TAIL = 4,
+    CTL_BEAT = 6,
+    CTL_WORDS = 8
+};
+
+struct ServeArgs {
+    SearchArgs base;           // per-workgroup scratch: ev[w], results[w], slices by ev[w]
+    const ServeDesc* ring;     // pinned, ring_mask + 1 entries
+    uint32_t ring_mask;
+    const uint8_t* in_arena;   // pinned, in_bytes
+    clf_track* out_arena;      // pinned, out_bytes
+    uint64_t in_bytes, out_bytes;
+    CompRec* comp;             // pinned, comp_mask + 1 entries
+    uint32_t comp_mask;
+    const uint32_t* host_stop; // pinned: nonzero asks the kernel to finish
+    uint32_t* ctl;             // device, CTL_WORDS
+    uint32_t* mailbox;         // device, one per workgroup: last descriptor + 1 handed to it
+    uint64_t* idle_ring;       // device: (position + 1) << 32 | workgroup of idle workers
+    uint32_t idle_mask;
+    int64_t wg_blob_bytes;     // scratch per workgroup
+    int64_t wg_hits;
+    int64_t wg_cap;
+    uint64_t idle_ticks;       // wall-clock ticks without work before leaving
+};
+
 }  // namespace clf

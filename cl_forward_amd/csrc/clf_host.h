@@ -21,6 +21,8 @@ hipError_t launch_pack(const EventDesc* ev, const EventResult* res, int n, int64
                        const clf_track* tracks, uint32_t* packed, hipStream_t stream);
 hipError_t occupancy(int block, int lds_hits, int* blocks_per_cu);
 hipError_t launch_probe_copy(const void* src, void* dst, int64_t bytes, hipStream_t stream);
+hipError_t launch_serve(const ServeArgs& a, int block, int lds_hits, int grid, hipStream_t stream);
+hipError_t serve_occupancy(int block, int lds_hits, int* blocks_per_cu);
 
 // Error reporting: the message is kept per thread for clf_last_error().
 clf_status fail(clf_status st, const std::string& msg);
@@ -59,10 +61,12 @@ struct DevBuf {
     }
 };
 
+// Pinned host memory.  `flags` as for hipHostMalloc; hipHostMallocCoherent
+// gives memory that a running kernel and the host may both access.
 struct PinnedBuf {
     void* p = nullptr;
     size_t cap = 0;
-    hipError_t ensure(size_t bytes) {
+    hipError_t ensure(size_t bytes, unsigned flags = hipHostMallocDefault) {
         if (bytes == 0) bytes = 256;
         if (bytes <= cap) return hipSuccess;
         if (p) {
@@ -70,7 +74,7 @@ struct PinnedBuf {
             p = nullptr;
             cap = 0;
         }
-        hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+        hipError_t e = hipHostMalloc(&p, bytes, flags);
         if (e == hipSuccess) cap = bytes;
         return e;
     }

@@ -147,6 +147,10 @@ struct Shared {
     int16_t v_stop[kMaxVisits * CLF_SEED_ROWS];
     unsigned long long v_best[kMaxVisits];
     unsigned long long prof[kProfSlots];
+    // resident serving: the claimed descriptor
+    uint64_t sv_in, sv_out;
+    int32_t sv_n, sv_s, sv_req, sv_cap;
+    uint32_t sv_d;
 };
 
 template <int B, int C>
@@ -725,7 +729,6 @@ template <bool L, int B, int C>
 __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
                                          const UsedFlags& used, clf_track* tracks, uint4* weak,
                                          bool degenerate, int f, StepState& st, PhaseClock& pc) {
-    constexpr int W = Shared<B, C>::W;
     const int tid = threadIdx.x, wave = wave_id();
     int& n_cur = st.n_cur;
     int& n_tracks = st.n_tracks;
@@ -1146,6 +1149,243 @@ __global__ __launch_bounds__(B) void search_kernel(SearchArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Resident serving kernel
+// ---------------------------------------------------------------------------
+// Workgroup 0 runs one wave, the feeder; every other workgroup is a worker.
+// A worker announces that it is idle by appending its index to the idle
+// ring (one fetch-add, never retried) and then watches its own mailbox.  The
+// feeder follows the host's descriptor ring in pinned memory and pairs
+// published descriptors with idle workers in order, up to 64 per round trip,
+// writing each descriptor number into the worker's mailbox.  A worker reads
+// its descriptor, copies the event from the pinned input arena into its own
+// scratch, runs process_event, copies the tracks into the request's pinned
+// output region and appends a completion record.
+//
+// Termination: the feeder leaves when the host sets its stop word, or when
+// every published descriptor has been handed out and nothing has moved for
+// idle_ticks; it then sets CTL_QUIT.  A waiting worker leaves on CTL_QUIT
+// (after one more look at its mailbox), or when the feeder's heartbeat has
+// been silent for 4 x idle_ticks (the feeder is not running, so nothing can
+// be handed to it).  Every wait is bounded once the host stops publishing.
+
+// Polls are relaxed (no cache invalidation per poll); the one acquire per
+// event is the worker's check of its descriptor's sequence word.
+__device__ __forceinline__ uint32_t poll_agent(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t poll_sys(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+// index of the first clear bit of m (64 if none)
+__device__ __forceinline__ int first_clear(uint64_t m) {
+    return ~m == 0ull ? kWave : __builtin_ctzll(~m);
+}
+
+// The feeder wave.  While descriptors keep coming it checks the next 64
+// sequence words at once (one round trip over the host link); otherwise
+// only the next one.
+__device__ void serve_feeder(const ServeArgs& a) {
+    const int lane = lane_id();
+    uint32_t* ctl = a.ctl;
+    uint64_t* beat = reinterpret_cast<uint64_t*>(ctl + CTL_BEAT);
+    uint32_t t = poll_agent(&ctl[CTL_TAIL]);
+    uint32_t h = poll_agent(&ctl[CTL_HEAD]);
+    uint32_t ih = 0;  // idle ring read position (the ring is cleared per launch)
+    uint64_t last = wall_clock64();
+    bool wide = false;
+    for (;;) {
+        const uint64_t now = wall_clock64();
+        if (lane == 0) __hip_atomic_store(beat, now, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        bool moved = false;
+        {
+            const uint32_t k = wide ? (uint32_t)lane : 0u;
+            const bool ok = poll_sys(&a.ring[(t + k) & a.ring_mask].seq) == t + k + 1u;
+            const uint64_t m = ballot(ok);
+            const int n = wide ? first_clear(m) : (m != 0ull ? 1 : 0);
+            wide = n > 0;
+            if (n > 0) {
+                t += (uint32_t)n;
+                moved = true;
+            }
+        }
+        if (t != h) {
+            const uint64_t e = __hip_atomic_load(&a.idle_ring[(ih + (uint32_t)lane) & a.idle_mask],
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const bool idle = (uint32_t)(e >> 32) == ih + (uint32_t)lane + 1u;
+            const int ni = first_clear(ballot(idle));
+            const int na = min(ni, (int)(t - h));
+            if (lane < na)
+                __hip_atomic_store(&a.mailbox[(uint32_t)e], h + (uint32_t)lane + 1u,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (na > 0) {
+                h += (uint32_t)na;
+                ih += (uint32_t)na;
+                moved = true;
+            }
+        }
+        if (moved) {
+            last = now;
+            continue;
+        }
+        if (poll_sys(a.host_stop) != 0u) break;
+        if (h == t && now - last > a.idle_ticks) break;
+        __builtin_amdgcn_s_sleep(2);
+    }
+    if (lane == 0) {
+        __hip_atomic_store(&ctl[CTL_HEAD], h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ctl[CTL_TAIL], t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ctl[CTL_QUIT], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+template <int B, int C>
+__device__ void serve_one(const ServeArgs& a, int w, Shared<B, C>& sh, uint64_t t_claim) {
+    const int tid = threadIdx.x;
+    const uint64_t in_off = sh.sv_in, out_off = sh.sv_out;
+    const int N = sh.sv_n, S = sh.sv_s, req = sh.sv_req, out_cap = sh.sv_cap;
+    const int64_t bytes = 8 + 12 * (int64_t)S + 16 * (int64_t)N;
+    // the host checked these at submit; a descriptor outside them is
+    // answered with CLF_FLAG_INVALID instead of being read
+    const int64_t cap = CLF_EVENT_TRACK_CAPACITY(N, S);
+    const bool fits = N >= 0 && S >= 0 && S <= CLF_MAX_SENSORS && N <= a.wg_hits &&
+                      bytes <= a.wg_blob_bytes && cap <= a.wg_cap && out_cap >= 0 &&
+                      in_off + (uint64_t)bytes <= a.in_bytes &&
+                      out_off + (uint64_t)out_cap * sizeof(clf_track) <= a.out_bytes;
+    // the workgroup's own scratch (SearchArgs holds it as read-only input)
+    uint8_t* blob = const_cast<uint8_t*>(a.base.blobs) + (size_t)w * (size_t)a.wg_blob_bytes;
+    if (fits) {
+        // copy-in from pinned host memory, four 16-byte loads in flight per lane
+        const uint4* src = reinterpret_cast<const uint4*>(a.in_arena + in_off);
+        uint4* dst = reinterpret_cast<uint4*>(blob);
+        const int64_t n16 = bytes >> 4;
+        int64_t i = tid;
+        for (; i + 3 * B < n16; i += 4 * B) {
+            const uint4 v0 = src[i], v1 = src[i + B], v2 = src[i + 2 * B], v3 = src[i + 3 * B];
+            dst[i] = v0;
+            dst[i + B] = v1;
+            dst[i + 2 * B] = v2;
+            dst[i + 3 * B] = v3;
+        }
+        for (; i < n16; i += B) dst[i] = src[i];
+        const uint32_t* s4 = reinterpret_cast<const uint32_t*>(a.in_arena + in_off);
+        uint32_t* d4 = reinterpret_cast<uint32_t*>(blob);
+        for (int64_t k = n16 * 4 + tid; k < bytes / 4; k += B) d4[k] = s4[k];
+    }
+    if (tid == 0) {
+        EventDesc ed;
+        ed.blob_off = (int64_t)w * a.wg_blob_bytes;
+        ed.hit_off = (int64_t)w * a.wg_hits;
+        ed.track_off = (int64_t)w * a.wg_cap;
+        ed.n_hits = fits ? N : -1;
+        ed.n_sensors = fits ? S : 0;
+        const_cast<EventDesc*>(a.base.ev)[w] = ed;
+    }
+    __threadfence_block();
+    __syncthreads();
+    const uint64_t t_copied = wall_clock64();
+    process_event<B, C>(a.base, w, sh);
+    __threadfence_block();
+    __syncthreads();
+    const uint64_t t_searched = wall_clock64();
+    const EventResult r = a.base.results[w];
+    const bool spill = fits && r.n_tracks > out_cap;
+    if (fits && !spill) {
+        // copy-out into the request's pinned output region
+        const uint32_t* ts = reinterpret_cast<const uint32_t*>(a.base.tracks + (size_t)w * a.wg_cap);
+        uint32_t* to = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(a.out_arena) + out_off);
+        const int64_t words = (int64_t)r.n_tracks * (int64_t)(sizeof(clf_track) / 4);
+        for (int64_t k = tid; k < words; k += B) to[k] = ts[k];
+    }
+    __threadfence_system();
+    __syncthreads();
+    if (tid == 0) {
+        const uint32_t p =
+            __hip_atomic_fetch_add(&a.ctl[CTL_DONE], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        CompRec* c = a.comp + (p & a.comp_mask);
+        c->req = req;
+        c->wg = w;
+        c->status = spill ? COMP_SPILL : COMP_OK;
+        c->res = r;
+        c->t_claim = t_claim;
+        c->t_copied = t_copied;
+        c->t_searched = t_searched;
+        c->t_written = wall_clock64();
+        __threadfence_system();
+        __hip_atomic_store(&c->seq, p + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+template <int B, int C>
+__global__ __launch_bounds__(B) void serve_kernel(ServeArgs a) {
+    __shared__ Shared<B, C> sh;
+    if (blockIdx.x == 0) {
+        if (threadIdx.x < kWave) serve_feeder(a);
+        return;
+    }
+    const int w = blockIdx.x;
+    uint32_t* ctl = a.ctl;
+    const uint64_t* beat = reinterpret_cast<const uint64_t*>(ctl + CTL_BEAT);
+    uint32_t last_mb = 0;  // thread 0: the mailbox value already taken
+    for (;;) {
+        if (threadIdx.x == 0) {
+            int32_t got = -1;
+            uint32_t d = 0;
+            const uint32_t pos = __hip_atomic_fetch_add(&ctl[CTL_IDLE_TAIL], 1u, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&a.idle_ring[pos & a.idle_mask],
+                               ((uint64_t)(pos + 1u) << 32) | (uint64_t)(uint32_t)w,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint64_t t0 = wall_clock64();
+            int nap = 1;  // idle polls back off (64-cycle units, up to ~4k cycles)
+            for (;;) {
+                uint32_t v = poll_agent(&a.mailbox[w]);
+                if (v == last_mb && poll_agent(&ctl[CTL_QUIT]) != 0u) {
+                    // the feeder has left; it wrote every hand-out before CTL_QUIT
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    v = poll_agent(&a.mailbox[w]);
+                    if (v == last_mb) break;
+                }
+                if (v != last_mb) {
+                    last_mb = v;
+                    d = v - 1u;
+                    got = 1;
+                    break;
+                }
+                const uint64_t hb = __hip_atomic_load(beat, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint64_t since = hb > t0 ? hb : t0;
+                if ((int64_t)(wall_clock64() - since) > (int64_t)(4 * a.idle_ticks)) break;
+                for (int k = 0; k < nap; ++k) __builtin_amdgcn_s_sleep(4);
+                nap = nap < 16 ? 2 * nap : 16;
+            }
+            sh.prof[0] = wall_clock64();
+            if (got > 0) {
+                const ServeDesc* sd = a.ring + (d & a.ring_mask);
+                // acquire: the fields below were written before the sequence word
+                const bool seen =
+                    __hip_atomic_load(&sd->seq, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) == d + 1u;
+                // the descriptor fields in one round trip
+                const uint4 q0 = reinterpret_cast<const uint4*>(sd)[0];
+                const uint4 q1 = reinterpret_cast<const uint4*>(sd)[1];
+                sh.sv_in = (uint64_t)q0.x | ((uint64_t)q0.y << 32);
+                sh.sv_out = (uint64_t)q0.z | ((uint64_t)q0.w << 32);
+                sh.sv_n = seen ? (int32_t)q1.x : -1;  // not seen cannot happen: the feeder saw it
+                sh.sv_s = (int32_t)q1.y;
+                sh.sv_req = (int32_t)q1.z;
+                sh.sv_cap = (int32_t)q1.w;
+            }
+            sh.item = got;
+            sh.sv_d = d;
+        }
+        __syncthreads();
+        const int got = sh.item;
+        const uint64_t t_claim = sh.prof[0];
+        __syncthreads();
+        if (got < 0) break;
+        serve_one<B, C>(a, w, sh, t_claim);
+    }
+}
+
 // Writes zeros: the queue counters of a launch (a kernel, not a memset, so
 // that the sequence can be captured into a graph and replayed).
 __global__ void zero_i32_kernel(int32_t* p, int n) {
@@ -1231,6 +1471,27 @@ hipError_t launch_search(const SearchArgs& a, int block, int lds_hits, int grid,
         hipLaunchKernelGGL((search_kernel<B_, C_>), dim3(grid), dim3(B_), 0, stream, a); \
         return hipGetLastError();                                                          \
     }
+    CLF_FOR_EACH_CONFIG(CLF_CASE)
+#undef CLF_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_serve(const ServeArgs& a, int block, int lds_hits, int grid, hipStream_t stream) {
+#define CLF_CASE(B_, C_)                                                                  \
+    if (block == B_ && lds_hits == C_) {                                                  \
+        hipLaunchKernelGGL((serve_kernel<B_, C_>), dim3(grid), dim3(B_), 0, stream, a); \
+        return hipGetLastError();                                                         \
+    }
+    CLF_FOR_EACH_CONFIG(CLF_CASE)
+#undef CLF_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t serve_occupancy(int block, int lds_hits, int* blocks_per_cu) {
+#define CLF_CASE(B_, C_)                                                                    \
+    if (block == B_ && lds_hits == C_)                                                      \
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,                  \
+                                                            serve_kernel<B_, C_>, B_, 0);
     CLF_FOR_EACH_CONFIG(CLF_CASE)
 #undef CLF_CASE
     return hipErrorInvalidValue;

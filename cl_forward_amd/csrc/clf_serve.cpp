@@ -1,17 +1,26 @@
 // clf_serve.cpp — the serving loop of the C ABI (clf_server_* in clf.h).
 //
-// Submitters copy each event into a pinned input arena and queue it.  One
-// dispatcher thread owns every HIP call: whenever one of the slots (a
-// stream with resident device buffers sized for a full batch) is free, it
-// takes the longest ready prefix of the queue that fits the slot, copies
-// it to the device in one or two DMA transfers (arena order is queue
+// Submitters copy each event into a pinned input arena (a ring, freed in
+// allocation order) and queue it; any number of threads may submit.  Two
+// engines take it from there:
+//
+// Resident: a persistent kernel (serve_kernel) follows a descriptor ring in
+// pinned memory.  Submit writes the descriptor and publishes it with a
+// release store of its sequence word; a workgroup claims it, reads the event
+// from the arena over the host link, searches it and writes the tracks into
+// the request's region of a pinned output arena, then appends a completion
+// record.  One host thread collects completions in order and relaunches the
+// kernel when it has left for lack of work.
+//
+// Batches: a dispatcher thread owns every HIP call.  Whenever one of the
+// slots (a stream with resident device buffers sized for a full batch) is
+// free, it takes the longest ready prefix of the queue that fits the slot,
+// copies it to the device in one or two DMA transfers (arena order is queue
 // order, so a batch is one contiguous arena range unless the ring wraps),
-// and enqueues the same batched search as clf_batch_run.  Results come back
-// in two steps: per-event stats and track offsets, then exactly the packed
-// tracks into one of a pool of pinned output buffers (two per slot), so a
-// slot takes its next batch as soon as its tracks are on the host.  An
-// output buffer returns to the pool when every request whose tracks it
-// holds has been released.
+// and enqueues the same batched search as clf_batch_run.  Per-event stats
+// and track offsets come back first, then exactly the packed tracks, into
+// one of a pool of pinned output buffers (two per slot), so a slot takes its
+// next batch as soon as its tracks are on the host.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -43,25 +52,62 @@ constexpr int64_t kEventCapOverhead = 64 * 48 + 64;
 
 enum ReqState : int { R_FREE = 0, R_WRITING, R_PENDING, R_RUNNING, R_DONE };
 
+// A ring of bytes handed out in allocation order and returned in any order;
+// space comes back from the front of the allocation order.
+struct Ring {
+    struct Entry {
+        size_t taken;  // bytes taken, wrap gap included
+        bool freed;
+    };
+    size_t cap = 0, head = 0, used = 0;
+    std::deque<Entry> fifo;
+    uint64_t base_seq = 0;  // sequence number of fifo.front()
+
+    bool fits(size_t span, size_t& off, size_t& taken) const {
+        if (head + span <= cap) {
+            off = head;
+            taken = span;
+        } else {
+            off = 0;
+            taken = (cap - head) + span;
+        }
+        return used + taken <= cap;
+    }
+    uint64_t commit(size_t off, size_t span, size_t taken) {
+        head = off + span == cap ? 0 : off + span;
+        used += taken;
+        fifo.push_back({taken, false});
+        return base_seq + fifo.size() - 1;
+    }
+    void release(uint64_t seq) {
+        fifo[seq - base_seq].freed = true;
+        while (!fifo.empty() && fifo.front().freed) {
+            used -= fifo.front().taken;
+            fifo.pop_front();
+            ++base_seq;
+        }
+        if (fifo.empty()) head = 0, used = 0;
+    }
+};
+
 struct Request {
     uint64_t ticket = 0;
     uint64_t tag = 0;
     std::atomic<int> state{R_FREE};
-    size_t arena_off = 0;  // first byte of the blob in the arena
-    size_t span = 0;       // aligned blob bytes
-    uint64_t arena_seq = 0;
+    size_t in_off = 0;  // first byte of the blob in the input arena
+    size_t span = 0;    // aligned blob bytes
+    uint64_t in_seq = 0;
+    bool in_live = false;
+    size_t out_off = 0;  // resident: its region in the output arena
+    uint64_t out_seq = 0;
+    uint64_t desc = 0;   // resident: its descriptor number
     int32_t S = 0, N = 0;
     int64_t cap = 0;
     double cost = 0.0;
-    int out_buf = -1;  // output buffer holding its tracks once finished
+    int out_buf = -1;  // batches: output buffer holding its tracks once finished
+    int64_t out_cap = 0;             // resident: records its output region holds
+    std::vector<uint8_t> spill;      // resident: tracks of an event that overflowed it
     clf_served out{};
-};
-
-// An arena allocation in allocation order; space returns to the ring from
-// the front once the front's batch has been copied to the device.
-struct ArenaEntry {
-    size_t taken;  // bytes taken from the ring, wrap gap included
-    bool freed;
 };
 
 // A slot's batch: RUNNING (copies in, search, stats out), DRAINED (its input
@@ -92,34 +138,58 @@ struct clf_server {
     int device = 0, n_cu = 0;
     clf_launch_config cfg{};
     clf_server_config conf{};
-    int n_slots = 0;
-    size_t slot_blob_bytes = 0;
-    int64_t slot_cap = 0;
+    bool resident = true;
 
     PinnedBuf arena;
-    size_t arena_head = 0, arena_used = 0;
-    std::deque<ArenaEntry> arena_fifo;
-    uint64_t arena_base_seq = 0;  // sequence number of arena_fifo.front()
-
+    Ring in_ring;
     std::vector<Request> reqs;
     uint64_t req_mask = 0;
     int req_shift = 0;
     std::vector<int32_t> free_reqs;
     uint64_t next_seq = 1;
-    std::deque<int32_t> pending;
 
-    Slot slots[kMaxSlots];
-    std::vector<OutBuf> outs;
     std::mutex mu;
     std::condition_variable cv_work, cv_done, cv_completed;
-    std::thread dispatcher;
+    std::thread worker;  // dispatcher (batches) or completion collector (resident)
     bool stop = false;
     clf_status error = CLF_OK;
     std::string error_msg;
-
     bool collect = false;  // replay: finished requests also go to `completed`
     std::deque<int32_t> completed;
     int64_t n_batches = 0, n_batched_events = 0;
+
+    // batches engine
+    int n_slots = 0;
+    size_t slot_blob_bytes = 0;
+    int64_t slot_cap = 0;
+    std::deque<int32_t> pending;
+    Slot slots[kMaxSlots];
+    std::vector<OutBuf> outs;
+
+    // resident engine
+    PinnedBuf out_arena, ring_mem, comp_mem, stop_mem;
+    Ring out_ring;
+    uint32_t ring_mask = 0, comp_mask = 0;
+    uint64_t next_desc = 0;
+    std::vector<uint8_t> desc_busy;
+    std::atomic<uint64_t> published{0};
+    uint64_t consumed = 0;
+    uint32_t comp_next = 0;
+    DevBuf ctl, mailbox, idle_ring, wg_blobs, wg_desc, wg_cand, wg_used, wg_tracks, wg_ttf_a, wg_ttf_b, wg_weak,
+        wg_results;
+    int grid = 0;
+    uint32_t idle_mask = 0;
+    int64_t wg_blob_bytes = 0, wg_hits = 0, wg_cap = 0;
+    // side lane for events whose tracks overflow their output region
+    Resident spill_dev;
+    PinnedBuf spill_desc;
+    hipStream_t spill_stream = nullptr;
+    hipStream_t rstream = nullptr;
+    hipEvent_t ev_kernel = nullptr;
+    bool kernel_running = false;
+    // device phase sums over collected completions (wall-clock ticks)
+    uint64_t dev_in = 0, dev_search = 0, dev_out = 0, dev_n = 0;
+    double tick_ns = 10.0;
 };
 
 namespace {
@@ -130,18 +200,11 @@ Request* find_request(clf_server* s, uint64_t ticket) {
     return r.ticket == ticket ? &r : nullptr;
 }
 
-// Gives arena space back from the front of the allocation order.  Lock held.
-void arena_reclaim(clf_server* s) {
-    while (!s->arena_fifo.empty() && s->arena_fifo.front().freed) {
-        s->arena_used -= s->arena_fifo.front().taken;
-        s->arena_fifo.pop_front();
-        ++s->arena_base_seq;
+void free_input(clf_server* s, Request& r) {  // lock held
+    if (r.in_live) {
+        s->in_ring.release(r.in_seq);
+        r.in_live = false;
     }
-    if (s->arena_fifo.empty()) s->arena_head = 0, s->arena_used = 0;
-}
-
-void arena_free(clf_server* s, const Request& r) {
-    s->arena_fifo[r.arena_seq - s->arena_base_seq].freed = true;
 }
 
 // Ends every request that has not finished with `st` (device failure or
@@ -171,10 +234,31 @@ void fail_all(clf_server* s, clf_status st, const std::string& msg) {
     s->cv_completed.notify_all();
 }
 
+// ---------------------------------------------------------------------------
+// Batches engine
+// ---------------------------------------------------------------------------
+// Returns a finished request's output and its table entry.  Lock held.
+bool release_locked(clf_server* s, uint64_t ticket) {
+    Request* r = find_request(s, ticket);
+    if (!r || r->state.load() != R_DONE) return false;
+    if (r->out_buf >= 0) {
+        OutBuf& o = s->outs[r->out_buf];
+        if (--o.outstanding == 0) o.busy = false;
+    }
+    r->out_buf = -1;
+    if (s->resident && r->out.status == CLF_OK) s->out_ring.release(r->out_seq);
+    std::vector<uint8_t>().swap(r->spill);
+    free_input(s, *r);
+    r->ticket = 0;
+    r->state.store(R_FREE);
+    s->free_reqs.push_back((int32_t)(r - s->reqs.data()));
+    return true;
+}
+
 bool slot_busy(const Slot& sl) { return sl.phase != S_FREE; }
 
-// Picks the next batch for slot `k` and enqueues it.  Returns false when
-// nothing was launched.
+// Picks the next batch for slot `k` and enqueues it; `launched` tells
+// whether anything was.
 clf_status launch_batch(clf_server* s, int k, bool device_idle, bool& launched) {
     launched = false;
     Slot& sl = s->slots[k];
@@ -228,9 +312,9 @@ clf_status launch_batch(clf_server* s, int k, bool device_idle, bool& launched) 
     std::vector<double> cost(n);
     for (int i = 0; i < n; ++i) {
         const Request& r = s->reqs[sl.reqs[i]];
-        if (r.arena_off != prev_end) {
+        if (r.in_off != prev_end) {
             if (n_runs == 4) return fail(CLF_E_DEVICE, "serving: batch spans too many arena runs");
-            runs[n_runs++] = {r.arena_off, pos, 0};
+            runs[n_runs++] = {r.in_off, pos, 0};
         }
         runs[n_runs - 1].len += r.span;
         d[i].blob_off = (int64_t)pos;
@@ -239,7 +323,7 @@ clf_status launch_batch(clf_server* s, int k, bool device_idle, bool& launched) 
         d[i].n_hits = r.N;
         d[i].n_sensors = r.S;
         pos += r.span;
-        prev_end = r.arena_off + r.span;
+        prev_end = r.in_off + r.span;
         hit_off += r.N;
         track_off += r.cap;
         cost[i] = r.cost;
@@ -289,8 +373,7 @@ clf_status poll_slot(clf_server* s, int k, bool& progress) {
         sl.total = total;
         {
             std::lock_guard<std::mutex> lk(s->mu);
-            for (int32_t idx : sl.reqs) arena_free(s, s->reqs[idx]);
-            arena_reclaim(s);
+            for (int32_t idx : sl.reqs) free_input(s, s->reqs[idx]);
         }
         sl.phase = S_DRAINED;
         progress = true;
@@ -381,16 +464,16 @@ void dispatch_loop(clf_server* s) {
             }
         }
         if (st != CLF_OK) {
-            std::lock_guard<std::mutex> lk(s->mu);
-            fail_all(s, st, std::string("serving: ") + clf_last_error());
-        }
-        bool busy = false;
-        for (int k = 0; k < s->n_slots; ++k) busy = busy || slot_busy(s->slots[k]);
-        if (st != CLF_OK) {
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                fail_all(s, st, std::string("serving: ") + clf_last_error());
+            }
             // drain what is still on the device before leaving
             for (int k = 0; k < s->n_slots; ++k) (void)hipStreamSynchronize(s->slots[k].stream);
             return;
         }
+        bool busy = false;
+        for (int k = 0; k < s->n_slots; ++k) busy = busy || slot_busy(s->slots[k]);
         if (stopping && !busy) return;
         if (progress) continue;
         if (busy) {
@@ -403,29 +486,339 @@ void dispatch_loop(clf_server* s) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Resident engine
+// ---------------------------------------------------------------------------
+ServeArgs serve_args(clf_server* s) {
+    ServeArgs a{};
+    SearchArgs& b = a.base;
+    b.blobs = s->wg_blobs.as<uint8_t>();
+    b.ev = s->wg_desc.as<EventDesc>();
+    b.order = nullptr;
+    b.first_item = 0;
+    b.n_items = 0;
+    b.queue_head = nullptr;
+    b.grain = 1;
+    b.cand = s->wg_cand.as<int4>();
+    b.used = s->wg_used.as<uint8_t>();
+    b.tracks = s->wg_tracks.as<clf_track>();
+    b.ttf_a = s->wg_ttf_a.as<TtfEntry>();
+    b.ttf_b = s->wg_ttf_b.as<TtfEntry>();
+    b.weak = s->wg_weak.as<uint4>();
+    b.results = s->wg_results.as<EventResult>();
+    b.fill_only = 0;
+    b.prof = nullptr;
+    void* p = nullptr;
+    (void)hipHostGetDevicePointer(&p, s->ring_mem.p, 0);
+    a.ring = static_cast<const ServeDesc*>(p);
+    a.ring_mask = s->ring_mask;
+    (void)hipHostGetDevicePointer(&p, s->arena.p, 0);
+    a.in_arena = static_cast<const uint8_t*>(p);
+    (void)hipHostGetDevicePointer(&p, s->out_arena.p, 0);
+    a.out_arena = static_cast<clf_track*>(p);
+    a.in_bytes = s->arena.cap;
+    a.out_bytes = s->out_arena.cap;
+    (void)hipHostGetDevicePointer(&p, s->comp_mem.p, 0);
+    a.comp = static_cast<CompRec*>(p);
+    a.comp_mask = s->comp_mask;
+    (void)hipHostGetDevicePointer(&p, s->stop_mem.p, 0);
+    a.host_stop = static_cast<const uint32_t*>(p);
+    a.ctl = s->ctl.as<uint32_t>();
+    a.mailbox = s->mailbox.as<uint32_t>();
+    a.idle_ring = s->idle_ring.as<uint64_t>();
+    a.idle_mask = s->idle_mask;
+    a.wg_blob_bytes = s->wg_blob_bytes;
+    a.wg_hits = s->wg_hits;
+    a.wg_cap = s->wg_cap;
+    int rate_khz = 0;
+    (void)hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, s->device);
+    if (rate_khz <= 0) rate_khz = 100000;
+    s->tick_ns = 1.0e6 / (double)rate_khz;
+    a.idle_ticks = (uint64_t)s->conf.idle_us * (uint64_t)rate_khz / 1000ull;
+    return a;
+}
+
+clf_status launch_resident(clf_server* s) {
+    // per-launch state: QUIT, IDLE_TAIL, BEAT, the mailboxes and the idle ring
+    HIP_TRY(launch_zero_i32(s->ctl.as<int32_t>() + CTL_QUIT, 1, s->rstream));
+    HIP_TRY(launch_zero_i32(s->ctl.as<int32_t>() + CTL_IDLE_TAIL, CTL_WORDS - CTL_IDLE_TAIL, s->rstream));
+    HIP_TRY(launch_zero_i32(s->mailbox.as<int32_t>(), s->grid, s->rstream));
+    HIP_TRY(launch_zero_i32(s->idle_ring.as<int32_t>(), 2 * (int)(s->idle_mask + 1u), s->rstream));
+    const ServeArgs a = serve_args(s);
+    HIP_TRY(launch_serve(a, s->cfg.block_size, s->cfg.lds_hits, s->grid, s->rstream));
+    HIP_TRY(hipEventRecord(s->ev_kernel, s->rstream));
+    s->kernel_running = true;
+    std::lock_guard<std::mutex> lk(s->mu);
+    ++s->n_batches;  // for the resident engine: launches of the kernel
+    return CLF_OK;
+}
+
+// Searches an event again on the side stream, for a request whose tracks
+// did not fit its output region; the tracks go to the request's own heap
+// buffer.  Blocking; the event is still in the input arena.
+clf_status search_spilled(clf_server* s, Request& r) {
+    EventDesc* d = s->spill_desc.as<EventDesc>();
+    d->blob_off = 0;
+    d->hit_off = 0;
+    d->track_off = 0;
+    d->n_hits = r.N;
+    d->n_sensors = r.S;
+    int32_t* order = reinterpret_cast<int32_t*>(d + 1);
+    order[0] = 0;
+    hipStream_t st = s->spill_stream;
+    const Resident& dev = s->spill_dev;
+    HIP_TRY(hipMemcpyAsync(dev.blobs.p, s->arena.as<uint8_t>() + r.in_off, event_bytes(r.S, r.N),
+                           hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(dev.desc.p, d, sizeof(EventDesc), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(dev.order.p, order, sizeof(int32_t), hipMemcpyHostToDevice, st));
+    clf_status cs = enqueue_search(dev, 1, CLF_MODE_BATCHED, s->cfg, s->n_cu, st, nullptr, nullptr);
+    if (cs != CLF_OK) return cs;
+    int64_t pre[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(pre, dev.prefix.p, sizeof(pre), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (pre[1] < 0 || pre[1] > r.cap) return fail(CLF_E_DEVICE, "serving: inconsistent track count");
+    r.spill.resize(sizeof(clf_track) * (size_t)pre[1]);
+    if (pre[1] > 0)
+        HIP_TRY(hipMemcpy(r.spill.data(), dev.packed.p, r.spill.size(), hipMemcpyDeviceToHost));
+    return CLF_OK;
+}
+
+// Takes finished requests off the completion ring, in order: copies what is
+// there, then updates the requests under one lock.
+bool collect_completions(clf_server* s, std::vector<CompRec>& buf) {
+    CompRec* ring = s->comp_mem.as<CompRec>();
+    buf.clear();
+    for (;;) {
+        CompRec* c = &ring[s->comp_next & s->comp_mask];
+        if (__atomic_load_n(&c->seq, __ATOMIC_ACQUIRE) != s->comp_next + 1u) break;
+        buf.push_back(*c);
+        ++s->comp_next;
+        if (buf.size() >= 4096) break;
+    }
+    if (buf.empty()) return false;
+    std::vector<int32_t> spilled;
+    const int64_t t = now_ns();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (const CompRec& rec : buf) {
+            ++s->consumed;
+            ++s->n_batched_events;
+            s->dev_in += rec.t_copied - rec.t_claim;
+            s->dev_search += rec.t_searched - rec.t_copied;
+            s->dev_out += rec.t_written - rec.t_searched;
+            ++s->dev_n;
+            if (rec.req < 0 || (size_t)rec.req >= s->reqs.size()) continue;
+            Request& r = s->reqs[rec.req];
+            if (r.state.load() != R_PENDING) continue;  // already ended by a failure
+            if (rec.status == COMP_SPILL) {
+                std::memcpy(&r.out.stats, &rec.res, sizeof(clf_event_stats));
+                spilled.push_back(rec.req);
+                continue;
+            }
+            free_input(s, r);
+            s->desc_busy[r.desc & s->ring_mask] = 0;
+            r.out.ticket = r.ticket;
+            r.out.tag = r.tag;
+            r.out.status = CLF_OK;
+            r.out.batch_events = 1;
+            std::memcpy(&r.out.stats, &rec.res, sizeof(clf_event_stats));
+            r.out.n_tracks = rec.res.n_tracks;
+            r.out.tracks = s->out_arena.as<uint8_t>() + r.out_off;
+            r.out.t_launch_ns = 0;
+            r.out.t_done_ns = t;
+            r.state.store(R_DONE);
+            if (s->collect) s->completed.push_back(rec.req);
+        }
+    }
+    for (int32_t idx : spilled) {
+        Request& r = s->reqs[idx];
+        const clf_status st = search_spilled(s, r);
+        std::lock_guard<std::mutex> lk(s->mu);
+        free_input(s, r);
+        s->desc_busy[r.desc & s->ring_mask] = 0;
+        r.out.ticket = r.ticket;
+        r.out.tag = r.tag;
+        r.out.status = st;
+        r.out.batch_events = 1;
+        r.out.n_tracks = st == CLF_OK ? (int64_t)(r.spill.size() / sizeof(clf_track)) : 0;
+        r.out.tracks = st == CLF_OK ? r.spill.data() : nullptr;
+        r.out.t_launch_ns = 0;
+        r.out.t_done_ns = now_ns();
+        r.state.store(R_DONE);
+        if (s->collect) s->completed.push_back(idx);
+    }
+    s->cv_done.notify_all();
+    if (s->collect) s->cv_completed.notify_all();
+    return true;
+}
+
+void resident_loop(clf_server* s) {
+    (void)hipSetDevice(s->device);
+    std::vector<CompRec> buf;
+    buf.reserve(4096);
+    int64_t last_query = 0;
+    for (;;) {
+        bool progress = collect_completions(s, buf);
+        clf_status st = CLF_OK;
+        bool stopping, outstanding;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            stopping = s->stop;
+            outstanding = s->consumed < s->published.load();
+        }
+        // has the kernel left?  (asked at most every 100 us while it has work)
+        const int64_t now = now_ns();
+        if (s->kernel_running && (!outstanding || stopping || now - last_query > 100000)) {
+            last_query = now;
+            hipError_t q = hipEventQuery(s->ev_kernel);
+            if (q == hipSuccess) {
+                s->kernel_running = false;
+                progress = collect_completions(s, buf) || progress;
+                std::lock_guard<std::mutex> lk(s->mu);
+                outstanding = s->consumed < s->published.load();
+            } else if (q != hipErrorNotReady) {
+                st = hip_fail(q, "serving: resident kernel");
+            }
+        }
+        if (st == CLF_OK && !s->kernel_running && !stopping && outstanding) {
+            st = launch_resident(s);
+            progress = true;
+        }
+        if (st != CLF_OK) {
+            std::lock_guard<std::mutex> lk(s->mu);
+            fail_all(s, st, std::string("serving: ") + clf_last_error());
+            __atomic_store_n(s->stop_mem.as<uint32_t>(), 1u, __ATOMIC_RELEASE);
+        }
+        if ((stopping || st != CLF_OK) && !s->kernel_running) return;
+        if (st != CLF_OK) {
+            (void)hipStreamSynchronize(s->rstream);
+            s->kernel_running = false;
+            return;
+        }
+        if (progress) continue;
+        if (s->kernel_running || outstanding) {
+            std::this_thread::yield();
+            continue;
+        }
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_work.wait_for(lk, std::chrono::microseconds(1000));
+    }
+}
+
+clf_status create_resident(clf_server* s) {
+    const clf_server_config& c = s->conf;
+    int per_cu = 0;
+    HIP_TRY(serve_occupancy(s->cfg.block_size, s->cfg.lds_hits, &per_cu));
+    if (per_cu < 1) return fail(CLF_E_DEVICE, "serving kernel does not fit on a compute unit");
+    s->grid = per_cu * s->n_cu;  // every workgroup resident at once; 0 is the feeder
+    s->wg_hits = c.max_event_hits;
+    s->wg_cap = CLF_EVENT_TRACK_CAPACITY(s->wg_hits, CLF_MAX_SENSORS);
+    s->wg_blob_bytes = (int64_t)align_up(event_bytes(CLF_MAX_SENSORS, (int32_t)s->wg_hits), kBlobAlign);
+    const size_t g = (size_t)s->grid;
+    int ring_n = 1, comp_n = 1;
+    while (ring_n < 2 * c.max_requests) ring_n <<= 1;
+    while (comp_n < c.max_requests) comp_n <<= 1;
+    s->ring_mask = (uint32_t)ring_n - 1u;
+    s->comp_mask = (uint32_t)comp_n - 1u;
+    s->desc_busy.assign((size_t)ring_n, 0);
+    const unsigned coh = hipHostMallocCoherent | hipHostMallocMapped;
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t r) {
+        if (e == hipSuccess) e = r;
+    };
+    chk(s->out_arena.ensure((size_t)c.out_arena_bytes, coh));
+    chk(s->ring_mem.ensure(sizeof(ServeDesc) * (size_t)ring_n, coh));
+    chk(s->comp_mem.ensure(sizeof(CompRec) * (size_t)comp_n, coh));
+    chk(s->stop_mem.ensure(256, coh));
+    chk(s->ctl.ensure(sizeof(uint32_t) * CTL_WORDS));
+    int idle_n = 1;
+    while (idle_n < s->grid) idle_n <<= 1;
+    s->idle_mask = (uint32_t)idle_n - 1u;
+    chk(s->mailbox.ensure(sizeof(uint32_t) * g));
+    chk(s->idle_ring.ensure(sizeof(uint64_t) * (size_t)idle_n));
+    chk(s->spill_dev.ensure(1, (size_t)s->wg_blob_bytes, s->wg_hits, s->wg_cap));
+    chk(s->spill_desc.ensure(256));
+    chk(hipStreamCreateWithFlags(&s->spill_stream, hipStreamNonBlocking));
+    chk(s->wg_blobs.ensure(g * (size_t)s->wg_blob_bytes));
+    chk(s->wg_desc.ensure(sizeof(EventDesc) * g));
+    chk(s->wg_cand.ensure(sizeof(int4) * g * (size_t)s->wg_hits));
+    chk(s->wg_used.ensure(g * (size_t)s->wg_hits));
+    chk(s->wg_tracks.ensure(sizeof(clf_track) * g * (size_t)s->wg_cap));
+    chk(s->wg_ttf_a.ensure(sizeof(TtfEntry) * g * (size_t)s->wg_cap));
+    chk(s->wg_ttf_b.ensure(sizeof(TtfEntry) * g * (size_t)s->wg_cap));
+    chk(s->wg_weak.ensure(sizeof(uint4) * g * (size_t)s->wg_cap));
+    chk(s->wg_results.ensure(sizeof(EventResult) * g));
+    chk(hipStreamCreateWithFlags(&s->rstream, hipStreamNonBlocking));
+    chk(hipEventCreateWithFlags(&s->ev_kernel, hipEventDisableTiming));
+    if (e != hipSuccess) return hip_fail(e, "resident server allocation");
+    // touch every page once so that no request pays for it
+    std::memset(s->out_arena.p, 0, s->out_arena.cap);
+    std::memset(s->ring_mem.p, 0, s->ring_mem.cap);
+    std::memset(s->comp_mem.p, 0, s->comp_mem.cap);
+    std::memset(s->stop_mem.p, 0, s->stop_mem.cap);
+    s->out_ring.cap = s->out_arena.cap;
+    HIP_TRY(launch_zero_i32(s->ctl.as<int32_t>(), CTL_WORDS, s->rstream));
+    HIP_TRY(hipStreamSynchronize(s->rstream));
+    return CLF_OK;
+}
+
+clf_status create_batches(clf_server* s) {
+    const clf_server_config& c = s->conf;
+    s->n_slots = c.slots;
+    s->slot_blob_bytes = (size_t)(16 * c.max_batch_hits + kEventBlobOverhead * c.max_batch_events);
+    s->slot_cap = c.max_batch_hits + kEventCapOverhead * (int64_t)c.max_batch_events;
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t r) {
+        if (e == hipSuccess) e = r;
+    };
+    for (int k = 0; k < s->n_slots && e == hipSuccess; ++k) {
+        Slot& sl = s->slots[k];
+        chk(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
+        chk(hipEventCreateWithFlags(&sl.ev_run, hipEventDisableTiming));
+        chk(hipEventCreateWithFlags(&sl.ev_copy, hipEventDisableTiming));
+        chk(sl.dev.ensure(c.max_batch_events, s->slot_blob_bytes, c.max_batch_hits, s->slot_cap));
+        chk(sl.h_desc.ensure(sizeof(EventDesc) * (size_t)c.max_batch_events));
+        chk(sl.h_order.ensure(sizeof(int32_t) * (size_t)c.max_batch_events));
+        chk(sl.h_results.ensure(sizeof(EventResult) * (size_t)c.max_batch_events));
+        chk(sl.h_prefix.ensure(sizeof(int64_t) * ((size_t)c.max_batch_events + 1)));
+    }
+    s->outs = std::vector<OutBuf>(2 * (size_t)s->n_slots);
+    for (OutBuf& o : s->outs) chk(o.mem.ensure(sizeof(clf_track) * (size_t)(c.max_batch_hits / 4 + 1024)));
+    if (e != hipSuccess) return hip_fail(e, "server allocation");
+    return CLF_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
 void clf_server_default_config(clf_server_config* cfg) {
     if (!cfg) return;
+    cfg->engine = CLF_SERVE_RESIDENT;
+    cfg->max_requests = 65536;
+    cfg->arena_bytes = (int64_t)512 << 20;
+    cfg->out_arena_bytes = (int64_t)1 << 30;
+    cfg->max_event_hits = 16384;
+    cfg->idle_us = 5000;
     cfg->slots = 4;
     cfg->max_batch_events = 256;
     cfg->max_batch_hits = 1 << 20;
     cfg->max_wait_us = 100;
-    cfg->max_requests = 65536;
-    cfg->arena_bytes = (int64_t)512 << 20;
+    cfg->out_tracks_per_1k_hits = 500;
+    cfg->out_min_tracks = 256;
+    cfg->reserved = 0;
 }
 
 void clf_server_destroy(clf_server* s) {
     if (!s) return;
-    if (s->dispatcher.joinable()) {
+    if (s->worker.joinable()) {
         {
             std::lock_guard<std::mutex> lk(s->mu);
             s->stop = true;
         }
+        if (s->stop_mem.p) __atomic_store_n(s->stop_mem.as<uint32_t>(), 1u, __ATOMIC_RELEASE);
         s->cv_work.notify_all();
-        s->dispatcher.join();
+        s->worker.join();
     }
     {
         std::lock_guard<std::mutex> lk(s->mu);
@@ -445,6 +838,21 @@ void clf_server_destroy(clf_server* s) {
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
     for (OutBuf& o : s->outs) o.mem.release();
+    if (s->rstream) (void)hipStreamSynchronize(s->rstream);
+    if (s->spill_stream) (void)hipStreamSynchronize(s->spill_stream);
+    DevBuf* dev[] = {&s->ctl,       &s->mailbox,  &s->idle_ring, &s->wg_blobs,
+                     &s->wg_desc,   &s->wg_cand,  &s->wg_used,   &s->wg_tracks,
+                     &s->wg_ttf_a,  &s->wg_ttf_b, &s->wg_weak,   &s->wg_results};
+    for (DevBuf* d : dev) d->release();
+    s->spill_dev.release();
+    s->spill_desc.release();
+    if (s->spill_stream) (void)hipStreamDestroy(s->spill_stream);
+    if (s->ev_kernel) (void)hipEventDestroy(s->ev_kernel);
+    if (s->rstream) (void)hipStreamDestroy(s->rstream);
+    s->out_arena.release();
+    s->ring_mem.release();
+    s->comp_mem.release();
+    s->stop_mem.release();
     s->arena.release();
     delete s;
 }
@@ -455,10 +863,18 @@ clf_status clf_server_create(clf_context* ctx, const clf_server_config* cfg, clf
     clf_server_config c;
     clf_server_default_config(&c);
     if (cfg) c = *cfg;
-    if (c.slots < 1 || c.slots > kMaxSlots || c.max_batch_events < 1 ||
-        c.max_batch_events > (1 << 20) || c.max_batch_hits < 1 || c.max_wait_us < 0 ||
-        c.max_requests < 1 || c.max_requests > (1 << 24) || c.arena_bytes < (int64_t)(1 << 20))
+    if ((c.engine != CLF_SERVE_RESIDENT && c.engine != CLF_SERVE_BATCHES) || c.max_requests < 1 ||
+        c.max_requests > (1 << 22) || c.arena_bytes < (int64_t)(1 << 20))
         return fail(CLF_E_INVALID_ARGUMENT, "bad server configuration");
+    if (c.engine == CLF_SERVE_RESIDENT &&
+        (c.max_event_hits < 1 || c.max_event_hits > CLF_MAX_SENSOR_HITS * 64 || c.idle_us < 1 ||
+         c.idle_us > 10000000 || c.out_arena_bytes < (int64_t)(1 << 20) ||
+         c.out_tracks_per_1k_hits < 0 || c.out_min_tracks < 0))
+        return fail(CLF_E_INVALID_ARGUMENT, "bad resident server configuration");
+    if (c.engine == CLF_SERVE_BATCHES &&
+        (c.slots < 1 || c.slots > kMaxSlots || c.max_batch_events < 1 ||
+         c.max_batch_events > (1 << 20) || c.max_batch_hits < 1 || c.max_wait_us < 0))
+        return fail(CLF_E_INVALID_ARGUMENT, "bad batch server configuration");
     clf_launch_config lc;
     clf_context_get_config(ctx, &lc);
     const int device = context_device(ctx);
@@ -467,9 +883,8 @@ clf_status clf_server_create(clf_context* ctx, const clf_server_config* cfg, clf
     s->cfg = lc;
     s->n_cu = clf_context_num_cus(ctx);
     s->conf = c;
-    s->n_slots = c.slots;
-    s->slot_blob_bytes = (size_t)(16 * c.max_batch_hits + kEventBlobOverhead * c.max_batch_events);
-    s->slot_cap = c.max_batch_hits + kEventCapOverhead * (int64_t)c.max_batch_events;
+    s->resident = c.engine == CLF_SERVE_RESIDENT;
+    s->device = device;
     int shift = 0;
     while ((1ll << shift) < c.max_requests) ++shift;
     s->req_shift = shift;
@@ -478,29 +893,23 @@ clf_status clf_server_create(clf_context* ctx, const clf_server_config* cfg, clf
     s->free_reqs.reserve(s->reqs.size());
     for (int64_t i = (int64_t)c.max_requests - 1; i >= 0; --i) s->free_reqs.push_back((int32_t)i);
 
-    hipError_t e = hipSetDevice(s->device = device);
-    auto chk = [&](hipError_t r) {
-        if (e == hipSuccess) e = r;
-    };
-    chk(s->arena.ensure((size_t)c.arena_bytes));
-    for (int k = 0; k < s->n_slots && e == hipSuccess; ++k) {
-        Slot& sl = s->slots[k];
-        chk(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
-        chk(hipEventCreateWithFlags(&sl.ev_run, hipEventDisableTiming));
-        chk(hipEventCreateWithFlags(&sl.ev_copy, hipEventDisableTiming));
-        chk(sl.dev.ensure(c.max_batch_events, s->slot_blob_bytes, c.max_batch_hits, s->slot_cap));
-        chk(sl.h_desc.ensure(sizeof(EventDesc) * (size_t)c.max_batch_events));
-        chk(sl.h_order.ensure(sizeof(int32_t) * (size_t)c.max_batch_events));
-        chk(sl.h_results.ensure(sizeof(EventResult) * (size_t)c.max_batch_events));
-        chk(sl.h_prefix.ensure(sizeof(int64_t) * ((size_t)c.max_batch_events + 1)));
-    }
-    s->outs = std::vector<OutBuf>(2 * (size_t)s->n_slots);
-    for (OutBuf& o : s->outs) chk(o.mem.ensure(sizeof(clf_track) * (size_t)(c.max_batch_hits / 4 + 1024)));
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess)
+        e = s->arena.ensure((size_t)c.arena_bytes,
+                            s->resident ? hipHostMallocCoherent | hipHostMallocMapped
+                                        : hipHostMallocDefault);
     if (e != hipSuccess) {
         clf_server_destroy(s);
-        return hip_fail(e, "server allocation");
+        return hip_fail(e, "server arena");
     }
-    s->dispatcher = std::thread(dispatch_loop, s);
+    std::memset(s->arena.p, 0, s->arena.cap);  // touch every page once
+    s->in_ring.cap = s->arena.cap;
+    clf_status st = s->resident ? create_resident(s) : create_batches(s);
+    if (st != CLF_OK) {
+        clf_server_destroy(s);
+        return st;
+    }
+    s->worker = std::thread(s->resident ? resident_loop : dispatch_loop, s);
     *out = s;
     return CLF_OK;
 }
@@ -514,50 +923,75 @@ clf_status clf_server_submit(clf_server* s, const uint8_t* event, size_t size, u
     const size_t bytes = event_bytes(S, N);
     const size_t span = align_up(bytes, kBlobAlign);
     const int64_t cap = CLF_EVENT_TRACK_CAPACITY(N, S);
-    if (span > s->slot_blob_bytes || N > s->conf.max_batch_hits || cap > s->slot_cap)
+    const int64_t out_cap =
+        std::min<int64_t>(cap, (int64_t)s->conf.out_min_tracks +
+                                   (int64_t)N * s->conf.out_tracks_per_1k_hits / 1000);
+    const size_t out_span = align_up(sizeof(clf_track) * (size_t)std::max<int64_t>(out_cap, 1), kBlobAlign);
+    if (s->resident) {
+        if (N > s->wg_hits || cap > s->wg_cap || (int64_t)span > s->wg_blob_bytes)
+            return fail(CLF_E_CAPACITY, "event larger than the serving kernel's scratch");
+    } else if (span > s->slot_blob_bytes || N > s->conf.max_batch_hits || cap > s->slot_cap) {
         return fail(CLF_E_CAPACITY, "event larger than a serving batch");
+    }
     Request* r = nullptr;
+    uint64_t d = 0;
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->error != CLF_OK) return fail(s->error, s->error_msg);
         if (s->stop) return fail(CLF_E_INVALID_ARGUMENT, "server stopped");
         if (s->free_reqs.empty()) return fail(CLF_E_CAPACITY, "request table full");
-        const size_t acap = s->arena.cap;
-        size_t off, taken;
-        if (s->arena_head + span <= acap) {
-            off = s->arena_head;
-            taken = span;
-        } else {
-            off = 0;
-            taken = (acap - s->arena_head) + span;
+        size_t off, taken, ooff = 0, otaken = 0;
+        if (!s->in_ring.fits(span, off, taken)) return fail(CLF_E_CAPACITY, "input arena full");
+        if (s->resident) {
+            d = s->next_desc;
+            if (s->desc_busy[d & s->ring_mask]) return fail(CLF_E_CAPACITY, "descriptor ring full");
+            if (!s->out_ring.fits(out_span, ooff, otaken))
+                return fail(CLF_E_CAPACITY, "output arena full");
         }
-        if (s->arena_used + taken > acap) return fail(CLF_E_CAPACITY, "input arena full");
-        s->arena_head = off + span == acap ? 0 : off + span;
-        s->arena_used += taken;
         const int32_t idx = s->free_reqs.back();
         s->free_reqs.pop_back();
         r = &s->reqs[idx];
         r->ticket = (s->next_seq++ << s->req_shift) | (uint64_t)idx;
         r->tag = tag;
-        r->arena_off = off;
+        r->in_off = off;
         r->span = span;
-        r->arena_seq = s->arena_base_seq + s->arena_fifo.size();
-        s->arena_fifo.push_back({taken, false});
+        r->in_seq = s->in_ring.commit(off, span, taken);
+        r->in_live = true;
+        if (s->resident) {
+            r->out_off = ooff;
+            r->out_seq = s->out_ring.commit(ooff, out_span, otaken);
+            r->desc = d;
+            s->desc_busy[d & s->ring_mask] = 1;
+            ++s->next_desc;
+        }
         r->S = S;
         r->N = N;
         r->cap = cap;
+        r->out_cap = out_cap;
         r->out_buf = -1;
         r->out = clf_served{};
         r->state.store(R_WRITING);
-        s->pending.push_back(idx);
+        if (!s->resident) s->pending.push_back(idx);
         *ticket = r->ticket;
     }
-    std::memcpy(s->arena.as<uint8_t>() + r->arena_off, event, bytes);
+    std::memcpy(s->arena.as<uint8_t>() + r->in_off, event, bytes);
     r->cost = event_cost(event, S, N);
     r->out.t_submit_ns = now_ns();
     // a failure of the server may have ended the request meanwhile
     int expected = R_WRITING;
-    r->state.compare_exchange_strong(expected, R_PENDING, std::memory_order_release);
+    if (!r->state.compare_exchange_strong(expected, R_PENDING, std::memory_order_acq_rel))
+        return CLF_OK;
+    if (s->resident) {
+        ServeDesc* sd = s->ring_mem.as<ServeDesc>() + (d & s->ring_mask);
+        sd->in_off = r->in_off;
+        sd->out_off = r->out_off;
+        sd->n_hits = N;
+        sd->n_sensors = S;
+        sd->req = (int32_t)(r - s->reqs.data());
+        sd->out_cap = (int32_t)out_cap;
+        __atomic_store_n(&sd->seq, (uint32_t)(d + 1), __ATOMIC_RELEASE);
+        s->published.fetch_add(1);
+    }
     s->cv_work.notify_one();
     return CLF_OK;
 }
@@ -581,17 +1015,8 @@ clf_status clf_server_release(clf_server* s, uint64_t ticket) {
     if (!s) return fail(CLF_E_INVALID_ARGUMENT, "null argument");
     {
         std::lock_guard<std::mutex> lk(s->mu);
-        Request* r = find_request(s, ticket);
-        if (!r || r->state.load() != R_DONE)
+        if (!release_locked(s, ticket))
             return fail(CLF_E_INVALID_ARGUMENT, "ticket unknown or not finished");
-        if (r->out_buf >= 0) {
-            OutBuf& o = s->outs[r->out_buf];
-            if (--o.outstanding == 0) o.busy = false;
-        }
-        r->out_buf = -1;
-        r->ticket = 0;
-        r->state.store(R_FREE);
-        s->free_reqs.push_back((int32_t)(r - s->reqs.data()));
     }
     s->cv_work.notify_one();
     return CLF_OK;
@@ -643,7 +1068,18 @@ clf_status clf_server_replay(clf_server* s, int32_t n_pool, const uint8_t* const
         s->collect = true;
         s->completed.clear();
     }
-    const int64_t b0 = s->n_batches, e0 = s->n_batched_events;
+    int64_t b0, e0;
+    uint64_t din0, dsr0, dout0, dn0;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        b0 = s->n_batches;
+        e0 = s->n_batched_events;
+        din0 = s->dev_in;
+        dsr0 = s->dev_search;
+        dout0 = s->dev_out;
+        dn0 = s->dev_n;
+    }
+    std::atomic<int64_t> submit_ns{0}, digest_ns{0};
     const int64_t start = now_ns() + 2000000;  // 2 ms for the threads to start
     std::atomic<int64_t> remaining{n};
     std::atomic<int64_t> max_lag{0};
@@ -664,6 +1100,7 @@ clf_status clf_server_replay(clf_server* s, int32_t n_pool, const uint8_t* const
             uint64_t ticket = 0;
             const int p = pick[i];
             clf_status st = clf_server_submit(s, pool[p], sizes[p], (uint64_t)i, &ticket);
+            submit_ns.fetch_add(now_ns() - now);
             if (st != CLF_OK) {
                 rec[i].status = st;
                 if (remaining.fetch_sub(1) == 1) {
@@ -674,32 +1111,44 @@ clf_status clf_server_replay(clf_server* s, int32_t n_pool, const uint8_t* const
         }
     };
     auto collector = [&]() {
+        std::vector<clf_served> got;
+        got.reserve(64);
         for (;;) {
-            int32_t idx;
-            clf_served o;
+            got.clear();
             {
                 std::unique_lock<std::mutex> lk(s->mu);
                 s->cv_completed.wait(lk, [&] { return !s->completed.empty() || remaining.load() <= 0; });
                 if (s->completed.empty()) return;
-                idx = s->completed.front();
-                s->completed.pop_front();
-                o = s->reqs[idx].out;
+                while (!s->completed.empty() && got.size() < 64) {
+                    got.push_back(s->reqs[s->completed.front()].out);
+                    s->completed.pop_front();
+                }
             }
-            const int64_t i = (int64_t)o.tag;
-            const int64_t arr = start + arrival_ns[i];
-            clf_replay_record& r = rec[i];
-            r.status = o.status;
-            r.latency_ns = o.t_done_ns - arr;
-            r.wait_ns = o.t_launch_ns > 0 ? o.t_launch_ns - arr : -1;
-            r.n_tracks = (int32_t)o.n_tracks;
-            r.flags = o.stats.flags;
-            r.batch_events = o.batch_events;
-            r.digest = o.status == CLF_OK ? clf_tracks_digest(o.tracks, o.n_tracks) : 0;
+            const int64_t c0 = now_ns();
+            int64_t ld_max = 0;
+            for (const clf_served& o : got) {
+                const int64_t i = (int64_t)o.tag;
+                const int64_t arr = start + arrival_ns[i];
+                clf_replay_record& r = rec[i];
+                r.status = o.status;
+                r.latency_ns = o.t_done_ns - arr;
+                r.wait_ns = o.t_launch_ns > 0 ? o.t_launch_ns - arr : -1;
+                r.n_tracks = (int32_t)o.n_tracks;
+                r.flags = o.stats.flags;
+                r.batch_events = o.batch_events;
+                r.digest = o.status == CLF_OK ? clf_tracks_digest(o.tracks, o.n_tracks) : 0;
+                ld_max = std::max(ld_max, o.t_done_ns);
+            }
             int64_t ld = last_done.load();
-            while (o.t_done_ns > ld && !last_done.compare_exchange_weak(ld, o.t_done_ns)) {
+            while (ld_max > ld && !last_done.compare_exchange_weak(ld, ld_max)) {
             }
-            (void)clf_server_release(s, o.ticket);
-            if (remaining.fetch_sub(1) == 1) {
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                for (const clf_served& o : got) (void)release_locked(s, o.ticket);
+            }
+            s->cv_work.notify_one();
+            digest_ns.fetch_add(now_ns() - c0);
+            if (remaining.fetch_sub((int64_t)got.size()) == (int64_t)got.size()) {
                 std::lock_guard<std::mutex> lk(s->mu);
                 s->cv_completed.notify_all();
             }
@@ -743,6 +1192,17 @@ clf_status clf_server_replay(clf_server* s, int32_t n_pool, const uint8_t* const
         sum->mean_batch_events =
             sum->batches > 0 ? (double)(s->n_batched_events - e0) / (double)sum->batches : 0.0;
         sum->max_submit_lag_ns = (double)max_lag.load();
+        const double dn = (double)(s->dev_n - dn0);
+        const double tk = s->tick_ns * 1e-3;
+        sum->dev_copy_in_us = dn > 0 ? (double)(s->dev_in - din0) * tk / dn : 0.0;
+        sum->dev_search_us = dn > 0 ? (double)(s->dev_search - dsr0) * tk / dn : 0.0;
+        sum->dev_copy_out_us = dn > 0 ? (double)(s->dev_out - dout0) * tk / dn : 0.0;
+        const double busy_us =
+            (double)((s->dev_in - din0) + (s->dev_search - dsr0) + (s->dev_out - dout0)) * tk;
+        const double workers = s->grid > 1 ? (double)(s->grid - 1) : 1.0;
+        sum->dev_busy_frac = s->resident && sum->wall_s > 0 ? busy_us * 1e-6 / (workers * sum->wall_s) : 0.0;
+        sum->host_submit_us = n > 0 ? (double)submit_ns.load() * 1e-3 / (double)n : 0.0;
+        sum->host_digest_us = lat.empty() ? 0.0 : (double)digest_ns.load() * 1e-3 / (double)lat.size();
     }
     return CLF_OK;
 }

@@ -159,30 +159,52 @@ clf_status clf_probe_copy(clf_context* ctx, int64_t bytes, int32_t reps, float* 
 
 /* ---- Serving ----------------------------------------------------------
  * A server accepts events one at a time and returns each event's tracks
- * as soon as the batch it joined has finished.  The reference has no
- * serving loop: its Gaudi offload handler calls gpuPixelSearchByTriplet
- * (GpuPixelSearchByTriplet.cpp:29-37) once per request with a blocking
- * upload, launch and read-back (KernelInvoker.cpp:114-206).  Here a
- * dispatcher thread keeps `slots` batches in flight, each on its own stream
- * with resident device buffers, and forms the next batch from everything
- * that arrived while the others ran (continuous batching).  Results are
- * bit-identical to clf_search on the same event.
+ * as soon as they are ready.  The reference has no serving loop: its Gaudi
+ * offload handler calls gpuPixelSearchByTriplet (GpuPixelSearchByTriplet.cpp
+ * :29-37) once per request with a blocking upload, launch and read-back
+ * (KernelInvoker.cpp:114-206).  Results are bit-identical to clf_search on
+ * the same event.  Two engines:
+ *
+ *  CLF_SERVE_RESIDENT  a persistent kernel stays on the device while there
+ *    is work: submit publishes a descriptor in pinned memory, one wave of
+ *    the kernel follows the descriptor ring, workgroups claim events one by
+ *    one, read the event from the pinned input arena, search it and write
+ *    its tracks straight into the request's pinned output region, then
+ *    append a completion record.  A host thread collects completions and
+ *    relaunches the kernel after it has left for lack of work (idle_us).
+ *  CLF_SERVE_BATCHES   a dispatcher thread keeps `slots` batches in flight,
+ *    each on its own stream with resident device buffers, and forms the
+ *    next batch from everything that arrived meanwhile; tracks come back
+ *    when the whole batch has finished.
  *
  * Threading: clf_server_submit, _wait and _release may be called from any
- * number of threads.  Admission: a submit that would exceed the pinned
- * input arena, the request table or a slot's capacity returns
+ * number of threads.  Admission: a submit that would exceed a pinned arena,
+ * the request table or the engine's per-event or per-batch limits returns
  * CLF_E_CAPACITY at once (nothing is queued). */
 typedef struct clf_server clf_server;
 
+typedef enum clf_serve_engine { CLF_SERVE_RESIDENT = 0, CLF_SERVE_BATCHES = 1 } clf_serve_engine;
+
 typedef struct clf_server_config {
-    int32_t slots;            /* batches in flight (streams), 1..8              */
-    int32_t max_batch_events; /* events per batch                               */
-    int64_t max_batch_hits;   /* hits per batch; sizes each slot's scratch      */
-    int32_t max_wait_us;      /* while another batch runs, a batch waits up to
-                                 this long to fill; an idle device launches at
-                                 once                                           */
+    int32_t engine;           /* clf_serve_engine                               */
     int32_t max_requests;     /* submitted and not yet released                 */
     int64_t arena_bytes;      /* pinned input arena                             */
+    int64_t out_arena_bytes;  /* resident: pinned output arena                  */
+    int32_t max_event_hits;   /* resident: largest event (workgroup scratch)    */
+    int32_t idle_us;          /* resident: the kernel leaves after this long
+                                 without work                                   */
+    int32_t slots;            /* batches: batches in flight (streams), 1..8     */
+    int32_t max_batch_events; /* batches: events per batch                      */
+    int64_t max_batch_hits;   /* batches: hits per batch; sizes each slot       */
+    int32_t max_wait_us;      /* batches: while another batch runs, a batch
+                                 waits up to this long to fill; an idle device
+                                 launches at once                               */
+    int32_t out_tracks_per_1k_hits; /* resident: output region of a request,
+                                 out_min_tracks + N * this / 1000 records (at
+                                 most the capacity bound); an event with more
+                                 tracks is searched again on a side stream     */
+    int32_t out_min_tracks;
+    int32_t reserved;
 } clf_server_config;
 
 /* A finished request.  `tracks` points into the server's pinned output and
@@ -191,23 +213,26 @@ typedef struct clf_served {
     uint64_t ticket;
     uint64_t tag;             /* the submitter's tag                            */
     int32_t status;           /* CLF_OK, or the error that ended the request    */
-    int32_t batch_events;     /* size of the batch the event ran in             */
+    int32_t batch_events;     /* batches: size of its batch; resident: 1        */
     clf_event_stats stats;
     int64_t n_tracks;
     const uint8_t* tracks;    /* n_tracks * 100 bytes                           */
     int64_t t_submit_ns;      /* steady clock: submit returned                  */
-    int64_t t_launch_ns;      /* its batch was enqueued                         */
+    int64_t t_launch_ns;      /* batches: its batch was enqueued; resident: 0   */
     int64_t t_done_ns;        /* its tracks were visible on the host            */
 } clf_served;
 
-/* Fills *cfg with the defaults (4 slots, 256 events / 2^20 hits per batch,
- * 100 us, 65536 requests, 512 MiB arena). */
+/* Fills *cfg with the defaults: resident engine, 65536 requests, 512 MiB
+ * input and 1 GiB output arenas, events up to 16384 hits, 5000 us idle,
+ * output regions of 256 + N/2 records; for the batch engine 4 slots of 256
+ * events / 2^20 hits, 100 us. */
 void clf_server_default_config(clf_server_config* cfg);
 /* Starts a server on the context's device with the context's launch
  * configuration.  Allocates every buffer up front. */
 clf_status clf_server_create(clf_context* ctx, const clf_server_config* cfg, clf_server** out);
-/* Stops the dispatcher after the requests in flight have finished and frees
- * everything; outstanding tracks pointers become invalid. */
+/* Stops the server after the requests in flight have finished and frees
+ * everything; no other call on it may be in progress.  Requests not yet
+ * finished end with CLF_E_INVALID_ARGUMENT. */
 void clf_server_destroy(clf_server* server);
 /* Copies one event into the input arena and queues it.  A malformed blob is
  * rejected with CLF_E_INVALID_ARGUMENT. */
@@ -226,7 +251,8 @@ clf_status clf_server_release(clf_server* server, uint64_t ticket);
  * counted from the scheduled arrival to the result visible on the host. */
 typedef struct clf_replay_record {
     int64_t latency_ns;       /* -1 if rejected                                 */
-    int64_t wait_ns;          /* scheduled arrival to batch launch              */
+    int64_t wait_ns;          /* batches: scheduled arrival to batch launch;
+                                 resident: -1                                   */
     uint64_t digest;
     int32_t n_tracks;
     int32_t status;           /* CLF_OK, or why the submit or the run failed    */
@@ -242,6 +268,12 @@ typedef struct clf_replay_summary {
     int64_t batches;
     double mean_batch_events;
     double max_submit_lag_ns; /* how late the submitters ran at worst           */
+    /* resident engine, device wall clock per event (means, microseconds):
+     * input copy over the host link, search, track copy and completion */
+    double dev_copy_in_us, dev_search_us, dev_copy_out_us;
+    double dev_busy_frac;     /* workgroup time with an event / (workers x wall) */
+    double host_submit_us;    /* mean duration of clf_server_submit             */
+    double host_digest_us;    /* mean time to digest and release one result     */
 } clf_replay_summary;
 
 clf_status clf_server_replay(clf_server* server, int32_t n_pool, const uint8_t* const* pool,

@@ -1,15 +1,20 @@
-"""Serving loop on the GPU: every served event's tracks must be byte-identical
-to the plain path's for the same event, whether it ran alone or inside a
-batch with others, and to the oracle's."""
+"""Serving loop on the GPU, both engines (the resident kernel and the
+batch dispatcher): every served event's tracks must be byte-identical to the
+plain path's for the same event, whether it ran alone or among others, and
+to the oracle's."""
 from __future__ import annotations
 
 import numpy as np
 import pytest
 
+from cl_forward_amd._lib import SERVE_BATCHES, SERVE_RESIDENT
 from cl_forward_amd.search import MODE_PLAIN, ClfError, Server, tracks_digest
 from tests.helpers import line_event
 
 pytestmark = pytest.mark.gpu
+
+ENGINES = pytest.mark.parametrize("engine", [SERVE_RESIDENT, SERVE_BATCHES],
+                                  ids=["resident", "batches"])
 
 
 @pytest.fixture(scope="module")
@@ -18,19 +23,24 @@ def plain_ref(gpu_ctx, plain51_blobs):
     return outs, stats
 
 
-def test_serve_single_matches_plain_and_oracle(gpu_ctx, oracle_mod, single_blob):
+@ENGINES
+def test_serve_single_matches_plain_and_oracle(gpu_ctx, oracle_mod, single_blob, engine):
     ref, _ = gpu_ctx.search([single_blob], MODE_PLAIN)
     want = oracle_mod.search_event(single_blob)[0]
-    with Server(gpu_ctx, slots=2, max_batch_events=16, max_batch_hits=1 << 16) as srv:
+    with Server(gpu_ctx, engine=engine, slots=2, max_batch_events=16,
+                max_batch_hits=1 << 16) as srv:
         t = srv.submit(single_blob, tag=7)
         raw, stats, timing = srv.wait(t, timeout_ms=20000)
     assert raw == ref[0] == want
     assert stats["n_tracks"] == len(raw) // 100
     assert timing["batch_events"] == 1
-    assert timing["submit_ns"] <= timing["launch_ns"] <= timing["done_ns"]
+    assert timing["submit_ns"] <= timing["done_ns"]
+    if engine == SERVE_BATCHES:
+        assert timing["submit_ns"] <= timing["launch_ns"] <= timing["done_ns"]
 
 
-def test_serve_mix_equals_alone(gpu_ctx, plain51_blobs, plain_ref, mcdata_sorted):
+@ENGINES
+def test_serve_mix_equals_alone(gpu_ctx, plain51_blobs, plain_ref, mcdata_sorted, engine):
     """All events submitted at once (they share batches) and then one at a
     time (each alone) give the plain path's bytes."""
     ref_outs, _ = plain_ref
@@ -38,20 +48,22 @@ def test_serve_mix_equals_alone(gpu_ctx, plain51_blobs, plain_ref, mcdata_sorted
     mc_ref, _ = gpu_ctx.search(mc, MODE_PLAIN)
     events = plain51_blobs + mc
     want = ref_outs + mc_ref
-    with Server(gpu_ctx, slots=3, max_batch_events=32, max_batch_hits=1 << 18,
+    with Server(gpu_ctx, engine=engine, slots=3, max_batch_events=32, max_batch_hits=1 << 18,
                 max_wait_us=200) as srv:
         tickets = [srv.submit(e, tag=i) for i, e in enumerate(events)]
         got = [srv.wait(t, timeout_ms=30000) for t in tickets]
-        assert any(g[2]["batch_events"] > 1 for g in got), "nothing was batched"
+        if engine == SERVE_BATCHES:
+            assert any(g[2]["batch_events"] > 1 for g in got), "nothing was batched"
         for i, (raw, stats, _) in enumerate(got):
             assert raw == want[i], f"event {i} differs in the mix"
         for i in (0, 5, 51, len(events) - 1):
             raw, _, timing = srv.wait(srv.submit(events[i]), timeout_ms=30000)
-            assert timing["batch_events"] == 1
+            assert timing["batch_events"] == 1  # alone
             assert raw == want[i], f"event {i} differs alone"
 
 
-def test_serve_replay_digests(gpu_ctx, plain51_blobs, plain_ref):
+@ENGINES
+def test_serve_replay_digests(gpu_ctx, plain51_blobs, plain_ref, engine):
     """Open-loop replay of 4000 arrivals drawn from the 51 events; every
     result's digest equals the plain path's."""
     ref_outs, ref_stats = plain_ref
@@ -60,7 +72,8 @@ def test_serve_replay_digests(gpu_ctx, plain51_blobs, plain_ref):
     n = 4000
     pick = rng.integers(0, len(plain51_blobs), n).astype(np.int32)
     arrival = np.cumsum(rng.exponential(1e9 / 100000.0, n)).astype(np.int64)
-    with Server(gpu_ctx, slots=4, max_batch_events=128, max_batch_hits=1 << 19) as srv:
+    with Server(gpu_ctx, engine=engine, slots=4, max_batch_events=128,
+                max_batch_hits=1 << 19) as srv:
         rec, summ = srv.replay(plain51_blobs, pick, arrival, submit_threads=2)
     ok = rec["status"] == 0
     assert summ["rejected"] == int((~ok).sum()) == 0
@@ -71,14 +84,15 @@ def test_serve_replay_digests(gpu_ctx, plain51_blobs, plain_ref):
     assert summ["batches"] >= 1 and summ["mean_batch_events"] >= 1.0
 
 
-def test_serve_small_arena_wraps(gpu_ctx, plain51_blobs, plain_ref):
+@ENGINES
+def test_serve_small_arena_wraps(gpu_ctx, plain51_blobs, plain_ref, engine):
     """A 1 MiB input arena holds about 25 of these events: the ring wraps
     many times, submits beyond it are refused with CLF_E_CAPACITY, and every
     accepted event is still exact."""
     ref_outs, _ = plain_ref
     want = np.array([tracks_digest(o) for o in ref_outs], np.uint64)
-    with Server(gpu_ctx, slots=2, max_batch_events=8, max_batch_hits=1 << 16,
-                arena_bytes=1 << 20) as srv:
+    with Server(gpu_ctx, engine=engine, slots=2, max_batch_events=8, max_batch_hits=1 << 16,
+                arena_bytes=1 << 20, out_arena_bytes=8 << 20) as srv:
         for rnd in range(3):
             for i, e in enumerate(plain51_blobs):
                 raw, _, _ = srv.wait(srv.submit(e), timeout_ms=30000)
@@ -94,12 +108,14 @@ def test_serve_small_arena_wraps(gpu_ctx, plain51_blobs, plain_ref):
     assert np.array_equal(rec["digest"][ok], want[pick[ok]])
 
 
-def test_serve_admission_errors(gpu_ctx, single_blob):
-    with Server(gpu_ctx, slots=1, max_batch_events=4, max_batch_hits=1000, max_requests=2) as srv:
+@ENGINES
+def test_serve_admission_errors(gpu_ctx, single_blob, engine):
+    with Server(gpu_ctx, engine=engine, slots=1, max_batch_events=4, max_batch_hits=1000,
+                max_event_hits=1000, max_requests=2) as srv:
         with pytest.raises(ClfError) as e:
             srv.submit(single_blob[:100])
         assert e.value.status == 1
-        with pytest.raises(ClfError) as e:  # 2,500 hits > max_batch_hits
+        with pytest.raises(ClfError) as e:  # 2,500 hits > the per-event limit
             srv.submit(single_blob)
         assert e.value.status == 5
         small = line_event([(1.0, 2.0, 0.01, -0.02), (-3.0, 1.0, -0.03, 0.01)],
@@ -115,3 +131,37 @@ def test_serve_admission_errors(gpu_ctx, single_blob):
         srv.wait(srv.submit(small), timeout_ms=20000)
         with pytest.raises(ClfError):
             srv.release(t1)  # already released
+
+
+def test_resident_idle_exit_and_relaunch(gpu_ctx, plain51_blobs, plain_ref):
+    """The resident kernel leaves after idle_us without work; a later submit
+    relaunches it and is served exactly."""
+    import time
+
+    ref_outs, _ = plain_ref
+    with Server(gpu_ctx, engine=SERVE_RESIDENT, idle_us=2000) as srv:
+        for rnd in range(3):
+            for i in (0, 7, 20):
+                raw, _, _ = srv.wait(srv.submit(plain51_blobs[i]), timeout_ms=30000)
+                assert raw == ref_outs[i], (rnd, i)
+            time.sleep(0.05)  # 25 x idle_us: the kernel has left
+
+
+def test_resident_spill_path_is_exact(gpu_ctx, plain51_blobs, plain_ref):
+    """Output regions of one or two records: nearly every event has more
+    tracks than that, is searched again on the side stream, and still comes
+    back byte-identical (alone and in a replay)."""
+    ref_outs, _ = plain_ref
+    want = np.array([tracks_digest(o) for o in ref_outs], np.uint64)
+    with Server(gpu_ctx, engine=SERVE_RESIDENT, out_min_tracks=1, out_tracks_per_1k_hits=0) as srv:
+        for i in (0, 3, 50):
+            raw, stats, _ = srv.wait(srv.submit(plain51_blobs[i]), timeout_ms=30000)
+            assert raw == ref_outs[i]
+            assert stats["n_tracks"] == len(raw) // 100
+        rng = np.random.default_rng(8)
+        n = 300
+        pick = rng.integers(0, len(plain51_blobs), n).astype(np.int32)
+        arrival = np.cumsum(rng.exponential(1e9 / 20000.0, n)).astype(np.int64)
+        rec, summ = srv.replay(plain51_blobs, pick, arrival, submit_threads=2)
+    assert summ["completed"] == n
+    assert np.array_equal(rec["digest"], want[pick])
