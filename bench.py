@@ -176,7 +176,7 @@ def main(argv=None):
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-batch", type=int, default=256)
     ap.add_argument("--serve-wait-us", type=int, default=100)
-    ap.add_argument("--serve-threads", type=int, default=4)
+    ap.add_argument("--serve-threads", type=int, default=8)
     ap.add_argument("--profile-phases", action="store_true",
                     help="only print the per-phase time breakdown of one plain and one batched run")
     a = ap.parse_args(argv)
