@@ -185,6 +185,15 @@ def load():
                                              ctypes.c_int32, ctypes.POINTER(ReplayRecord),
                                              ctypes.POINTER(ReplaySummary)]),
         "clf_tracks_digest": (ctypes.c_uint64, [u8p, ctypes.c_int64]),
+        "clf_server_submit_framed": (ctypes.c_int, [vp, u8p, ctypes.c_size_t, ctypes.c_uint64,
+                                                    ctypes.POINTER(ctypes.c_uint64)]),
+        "clf_unframe": (ctypes.c_int, [u8p, ctypes.c_size_t, ctypes.POINTER(u8p),
+                                       ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(u8p),
+                                       ctypes.POINTER(ctypes.c_size_t)]),
+        "clf_format_tracks": (ctypes.c_int, [u8p, ctypes.c_size_t, u8p, ctypes.c_int64,
+                                             ctypes.c_char_p, ctypes.c_size_t,
+                                             ctypes.POINTER(ctypes.c_size_t)]),
+        "clf_context_device_name": (ctypes.c_char_p, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -202,7 +211,8 @@ EXPORTED = (
     "clf_batch_run", "clf_batch_results", "clf_batch_time", "clf_search", "clf_presort_event",
     "clf_fill_candidates", "clf_probe_copy", "clf_batch_profile", "clf_server_default_config",
     "clf_server_create", "clf_server_destroy", "clf_server_submit", "clf_server_wait",
-    "clf_server_release", "clf_server_replay", "clf_tracks_digest",
+    "clf_server_release", "clf_server_replay", "clf_tracks_digest", "clf_server_submit_framed",
+    "clf_unframe", "clf_format_tracks", "clf_context_device_name",
 )
 
 

@@ -141,6 +141,7 @@ clf_status enqueue_search(const Resident& r, int32_t n, clf_mode mode,
 
 struct clf_context {
     int device = 0;
+    char name[256] = {0};
     hipStream_t stream = nullptr;
     int n_cu = 0;
     clf_launch_config cfg = {256, 0, 1, 1, 256};
@@ -247,6 +248,7 @@ clf_status clf_context_create(int device, clf_context** out) {
         return hip_fail(e, "hipGetDeviceProperties");
     }
     ctx->n_cu = prop.multiProcessorCount;
+    std::snprintf(ctx->name, sizeof(ctx->name), "%s", prop.name[0] ? prop.name : prop.gcnArchName);
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
@@ -290,6 +292,8 @@ clf_status clf_context_get_config(const clf_context* ctx, clf_launch_config* cfg
 }
 
 int clf_context_num_cus(const clf_context* ctx) { return ctx ? ctx->n_cu : 0; }
+
+const char* clf_context_device_name(const clf_context* ctx) { return ctx ? ctx->name : ""; }
 
 clf_status clf_context_synchronize(clf_context* ctx) {
     if (!ctx) return fail(CLF_E_INVALID_ARGUMENT, "null context");

@@ -996,6 +996,15 @@ clf_status clf_server_submit(clf_server* s, const uint8_t* event, size_t size, u
     return CLF_OK;
 }
 
+clf_status clf_server_submit_framed(clf_server* s, const uint8_t* frame, size_t size, uint64_t tag,
+                                    uint64_t* ticket) {
+    const uint8_t* data = nullptr;
+    size_t n = 0;
+    clf_status st = clf_unframe(frame, size, nullptr, nullptr, &data, &n);
+    if (st != CLF_OK) return st;
+    return clf_server_submit(s, data, n, tag, ticket);
+}
+
 clf_status clf_server_wait(clf_server* s, uint64_t ticket, int32_t timeout_ms, clf_served* out) {
     if (!s || !out) return fail(CLF_E_INVALID_ARGUMENT, "null argument");
     std::unique_lock<std::mutex> lk(s->mu);

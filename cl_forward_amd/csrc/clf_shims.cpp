@@ -1,8 +1,10 @@
 // clf_shims.cpp — the reference's C++ entry points (GpuPixelSearchByTriplet.h,
 // KernelInvoker.h) as thin shims over the C ABI of include/clf.h.
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <iomanip>
 #include <iostream>
 #include <mutex>
 #include <vector>
@@ -34,6 +36,57 @@ clf_context* context() {
     return g_ctx;
 }
 
+// The reference's timing report (KernelInvoker.cpp:136-193, 245-250, with
+// calcResults, Tools.cpp:82-107): four experiments of four timed runs of
+// the search kernel, then mean and standard deviation per experiment in
+// single precision.  Its experiments vary the local size in y (1..4), which
+// decides its results; here the results never depend on the launch, so the
+// four experiments are the workgroup sizes 64, 128, 256 and 512, printed as
+// (64, waves).  The configured launch is restored afterwards.
+struct TimingReport {
+    int waves[4] = {1, 2, 4, 8};
+    std::vector<float> ms[4];
+};
+
+void run_timing(clf_context* ctx, clf_batch* batch, TimingReport& rep) {
+    clf_launch_config saved;
+    clf_context_get_config(ctx, &saved);
+    for (int i = 0; i < 4; ++i) {
+        clf_launch_config c = saved;
+        c.block_size = 64 * rep.waves[i];
+        c.grid_size = 0;
+        if (clf_context_set_config(ctx, &c) != CLF_OK) continue;
+        std::cout << i + 1 << ": " << std::flush;
+        for (int j = 0; j < 4; ++j) {
+            clf_timing t;
+            clf_status s = clf_batch_time(ctx, batch, CLF_MODE_BATCHED, 0, 1, &t);
+            if (s != CLF_OK) fatal(s, "clf_batch_time");
+            rep.ms[i].push_back(t.ms_search);
+            std::cout << "." << std::flush;
+        }
+        std::cout << std::endl;
+    }
+    clf_context_set_config(ctx, &saved);
+}
+
+void print_timing(const TimingReport& rep) {
+    std::cout << std::endl << "Time averages:" << std::endl;
+    for (int i = 0; i < 4; ++i) {
+        const std::vector<float>& v = rep.ms[i];
+        if (v.empty()) continue;
+        float mean = 0.0f, variance = 0.0f;
+        for (float x : v) {
+            mean += x;
+            variance += x * x;
+        }
+        mean /= v.size();
+        variance = (variance / v.size()) - (mean * mean);
+        const float deviation = std::sqrt(variance);
+        std::cout << " nthreads (64, " << rep.waves[i] << "): " << mean << " ms (std dev "
+                  << deviation << ")" << std::endl;
+    }
+}
+
 int invoke_locked(int start, int count, const std::vector<const std::vector<uint8_t>*>& input,
                   std::vector<std::vector<uint8_t> >& output) {
     clf_context* ctx = context();
@@ -44,11 +97,14 @@ int invoke_locked(int start, int count, const std::vector<const std::vector<uint
         ptrs[i] = ev && !ev->empty() ? ev->data() : nullptr;
         sizes[i] = ev ? ev->size() : 0;
     }
-    if (g_verbosity >= 3) std::cout << "Invoking kernels on " << clf_version() << std::endl;
+    if (g_verbosity >= 3)
+        std::cout << "Invoking kernels on your " << clf_context_device_name(ctx) << std::endl;
     clf_result res;
     clf_batch* batch = nullptr;
     clf_status s = clf_search(ctx, CLF_MODE_BATCHED, count, ptrs.data(), sizes.data(), &res, &batch);
     if (s != CLF_OK) fatal(s, "clf_search");
+    TimingReport timing;
+    if (g_verbosity >= 3 && count > 0) run_timing(ctx, batch, timing);
     for (int i = 0; i < count; ++i) {
         const int64_t a = res.track_offset[i], b = res.track_offset[i + 1];
         std::vector<uint8_t>& out = output[start + i];
@@ -56,6 +112,7 @@ int invoke_locked(int start, int count, const std::vector<const std::vector<uint
         if (b > a) std::memcpy(out.data(), res.tracks + a * sizeof(clf_track), out.size());
     }
     clf_batch_destroy(batch);
+    if (g_verbosity >= 3 && count > 0) print_timing(timing);
     return 0;
 }
 
@@ -90,6 +147,7 @@ int gpuPixelSearchByTripletInvocation(const std::vector<const std::vector<uint8_
 
 int gpuPixelSearchByTriplet(const std::vector<const std::vector<uint8_t>*>& input,
                             std::vector<std::vector<uint8_t> >& output) {
+    std::cout << std::fixed << std::setprecision(2);  // GpuPixelSearchByTriplet.cpp:33-34
     g_verbosity = 0;
     return gpuPixelSearchByTripletInvocation(input, output);
 }
@@ -98,6 +156,7 @@ int independent_execute(const std::vector<std::vector<uint8_t> >& input,
                         std::vector<std::vector<uint8_t> >& output) {
     std::vector<const std::vector<uint8_t>*> converted(input.size());
     for (size_t i = 0; i < input.size(); ++i) converted[i] = &input[i];
+    std::cout << std::fixed << std::setprecision(2);  // GpuPixelSearchByTriplet.cpp:15-16
     g_verbosity = 3;
     // in place through the const reference, like preorder_by_x (Tools.cpp:57-63)
     for (size_t i = 0; i < input.size(); ++i) {

@@ -29,7 +29,7 @@ TRACK_BYTES = 100
 MAX_EVENTS_PER_LAUNCH = 16000  # GpuPixelSearchByTriplet.cpp:54
 
 __all__ = [
-    "Context", "Batch", "Server", "ClfError", "tracks_digest", "MODE_PLAIN", "MODE_BATCHED", "gpu_pixel_search_by_triplet",
+    "Context", "Batch", "Server", "ClfError", "tracks_digest", "format_tracks", "unframe", "MODE_PLAIN", "MODE_BATCHED", "gpu_pixel_search_by_triplet",
     "gpu_pixel_search_by_triplet_invocation", "independent_execute", "independent_post_execute",
     "invoke_parallel_search", "presort_event", "track_capacity",
 ]
@@ -89,6 +89,10 @@ class Context:
 
     def synchronize(self):
         _lib.check(self._L.clf_context_synchronize(self._h), "clf_context_synchronize")
+
+    @property
+    def device_name(self) -> str:
+        return (self._L.clf_context_device_name(self._h) or b"").decode(errors="replace")
 
     @property
     def num_cus(self) -> int:
@@ -220,6 +224,38 @@ class Batch:
         return {"ms_run": float(t.ms_run), "ms_search": float(t.ms_search), "reps": int(t.reps)}
 
 
+def format_tracks(event: bytes, tracks: bytes) -> str:
+    """The reference's results/<i>.out text of one event (clf_format_tracks):
+    printTrack lines for each 100-byte Track record of ``tracks``."""
+    L = _lib.load()
+    ev = (ctypes.c_uint8 * max(len(event), 1)).from_buffer_copy(bytes(event) or b"\0")
+    n = len(tracks) // TRACK_BYTES
+    tr = (ctypes.c_uint8 * max(len(tracks), 1)).from_buffer_copy(bytes(tracks) or b"\0")
+    need = ctypes.c_size_t()
+    _lib.check(L.clf_format_tracks(ev, len(event), tr, n, None, 0, ctypes.byref(need)),
+               "clf_format_tracks")
+    buf = ctypes.create_string_buffer(need.value + 1)
+    _lib.check(L.clf_format_tracks(ev, len(event), tr, n, buf, need.value + 1, ctypes.byref(need)),
+               "clf_format_tracks")
+    return buf.raw[: need.value].decode("ascii")
+
+
+def unframe(frame: bytes):
+    """(name, data) of one Gaudi offload frame (clf_unframe)."""
+    L = _lib.load()
+    buf = (ctypes.c_uint8 * max(len(frame), 1)).from_buffer_copy(bytes(frame) or b"\0")
+    name_p = ctypes.POINTER(ctypes.c_uint8)()
+    data_p = ctypes.POINTER(ctypes.c_uint8)()
+    nl = ctypes.c_int32()
+    ds = ctypes.c_size_t()
+    _lib.check(L.clf_unframe(buf, len(frame), ctypes.byref(name_p), ctypes.byref(nl),
+                             ctypes.byref(data_p), ctypes.byref(ds)), "clf_unframe")
+    base = ctypes.addressof(buf)
+    n_off = ctypes.cast(name_p, ctypes.c_void_p).value - base
+    d_off = ctypes.cast(data_p, ctypes.c_void_p).value - base
+    return bytes(frame[n_off:n_off + nl.value]), bytes(frame[d_off:d_off + ds.value])
+
+
 def tracks_digest(tracks: bytes) -> int:
     """clf_tracks_digest of packed Track records (len a multiple of 100)."""
     L = _lib.load()
@@ -273,6 +309,15 @@ class Server:
         t = ctypes.c_uint64()
         _lib.check(self._L.clf_server_submit(self._h, buf, len(b), int(tag), ctypes.byref(t)),
                    "clf_server_submit")
+        return int(t.value)
+
+    def submit_framed(self, frame: bytes, tag: int = 0) -> int:
+        """submit() of a request in the Gaudi offload framing."""
+        b = bytes(frame)
+        buf = (ctypes.c_uint8 * max(len(b), 1)).from_buffer_copy(b or b"\0")
+        t = ctypes.c_uint64()
+        _lib.check(self._L.clf_server_submit_framed(self._h, buf, len(b), int(tag), ctypes.byref(t)),
+                   "clf_server_submit_framed")
         return int(t.value)
 
     def wait(self, ticket: int, timeout_ms: int = -1, release: bool = True):

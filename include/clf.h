@@ -244,6 +244,13 @@ clf_status clf_server_wait(clf_server* server, uint64_t ticket, int32_t timeout_
                            clf_served* out);
 /* Gives a finished request's output back to the server. */
 clf_status clf_server_release(clf_server* server, uint64_t ticket);
+/* clf_server_submit of a request in the Gaudi offload framing, the wire
+ * format of the reference's .dat files and offload handler
+ * (IndependentEntrypoint.cpp:103-123): int32 name_len; name; int32
+ * data_size; data (bytes after the data are ignored).  The name is not
+ * checked. */
+clf_status clf_server_submit_framed(clf_server* server, const uint8_t* frame, size_t size,
+                                    uint64_t tag, uint64_t* ticket);
 
 /* Load replay: arrival i submits pool event pick[i] at arrival_ns[i] after
  * the start (open loop, from submit_threads threads), every result is
@@ -286,6 +293,28 @@ clf_status clf_server_replay(clf_server* server, int32_t n_pool, const uint8_t* 
  * words spread over the lanes in turn, then a final mix with the count.
  * Used to compare outputs of different paths without moving them. */
 uint64_t clf_tracks_digest(const uint8_t* tracks, int64_t n_tracks);
+
+/* Splits one Gaudi offload frame (IndependentEntrypoint.cpp:103-123):
+ * int32 name_len; char name[name_len]; int32 data_size; uint8 data[...].
+ * Pointers into `frame`; name and name_len may be null.  Unlike the
+ * reference reader, lengths are checked against `size`. */
+clf_status clf_unframe(const uint8_t* frame, size_t size, const uint8_t** name, int32_t* name_len,
+                       const uint8_t** data, size_t* data_size);
+
+/* The reference's per-event result text, results/<i>.out
+ * (KernelInvoker.cpp:209-243 with printTrack :254-275 and
+ * findClosestModule :284-303): for each track "Track #<n>, length <h>" and
+ * one line per hit with ID, hit index, module, x, y, z in default C++
+ * stream formatting, then an empty line.  The z -> module map is built as
+ * the reference builds it (sensor Zs, then every hit's truncated z mapped
+ * to the nearest key present, in hit order).  *len receives the text
+ * length; text (capacity cap, NUL-terminated) may be null to ask for it. */
+clf_status clf_format_tracks(const uint8_t* event, size_t size, const uint8_t* tracks,
+                             int64_t n_tracks, char* text, size_t cap, size_t* len);
+
+/* Name of the context's device (e.g. for the reference's
+ * "Invoking kernels on your <device>" line, KernelInvoker.cpp:145-147). */
+const char* clf_context_device_name(const clf_context* ctx);
 
 /* x-presort of one event in place, per sensor, with the reference's
  * partition sequence so that tied x keep its order (preorder_by_x,
