@@ -285,6 +285,21 @@ def main(argv=None):
                  "tracks_identical": p_outs == outs}
         log(f"plain: {pt['ms_run']:.3f} ms/step; identical tracks: {plain['tracks_identical']}")
 
+    # the boundary's own call (host buffers in, host buffers out): upload,
+    # batched run and read-back of the whole batch, as clf_search does
+    e2e = None
+    if not a.no_plain:
+        ctx.search(blobs[:8], MODE_BATCHED)  # warm-up of the allocation path
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            e_outs, _ = ctx.search(blobs, MODE_BATCHED)
+        dt = (time.perf_counter() - t0) / reps
+        e2e = {"ms_per_call": round(dt * 1e3, 3), "events_per_s": round(n_events / dt, 1),
+               "tracks_identical": e_outs == outs,
+               "what": "clf_search: allocate, upload, one batched run, read back (host in, host out)"}
+        log(f"end to end: {dt * 1e3:.3f} ms per call")
+
     # roofline of the search kernel: algorithmic bytes per launch
     # (input read once + output records written once) over its duration
     alg_bytes = in_bytes + TRACK_BYTES * n_tracks
@@ -336,6 +351,7 @@ def main(argv=None):
         "device_ms_per_step": round(dev["ms_run"], 4),
         "plain": plain,
         "batched_vs_plain": round(value / plain["events_per_s"], 2) if plain else None,
+        "end_to_end": e2e,
         "oracle_identical_events": oracle_identical,
         "roofline": roofline,
         "cpu_baseline": cpu,
