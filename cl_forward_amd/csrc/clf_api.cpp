@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -78,6 +79,14 @@ void Resident::release() {
     for (DevBuf* d : all) d->release();
 }
 
+void Resident::keep_larger(Resident& o) {
+    DevBuf* mine[] = {&blobs, &desc,  &order, &queue,   &cand,   &used,  &tracks,
+                      &ttf_a, &ttf_b, &weak,  &results, &prefix, &packed};
+    DevBuf* theirs[] = {&o.blobs, &o.desc,  &o.order, &o.queue,   &o.cand,   &o.used,  &o.tracks,
+                        &o.ttf_a, &o.ttf_b, &o.weak,  &o.results, &o.prefix, &o.packed};
+    for (int i = 0; i < 13; ++i) mine[i]->keep_larger(*theirs[i]);
+}
+
 SearchArgs Resident::args(int32_t n) const {
     SearchArgs a;
     a.blobs = blobs.as<uint8_t>();
@@ -146,6 +155,11 @@ struct clf_context {
     int n_cu = 0;
     clf_launch_config cfg = {256, 0, 1, 1, 256};
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+    // buffers of destroyed batches, reused by the next batch of this context
+    // (the one-call clf_search then allocates nothing in steady state)
+    std::mutex spare_mu;
+    Resident spare;
+    PinnedBuf spare_blobs, spare_results, spare_prefix, spare_packed;
 };
 
 struct clf_batch {
@@ -271,6 +285,11 @@ void clf_context_destroy(clf_context* ctx) {
     if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
     if (ctx->ev3) (void)hipEventDestroy(ctx->ev3);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    ctx->spare.release();
+    ctx->spare_blobs.release();
+    ctx->spare_results.release();
+    ctx->spare_prefix.release();
+    ctx->spare_packed.release();
     delete ctx;
 }
 
@@ -315,6 +334,15 @@ void clf_batch_destroy(clf_batch* b) {
         (void)hipStreamSynchronize(b->ctx->stream);
     }
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    if (b->ctx) {
+        clf_context* c = b->ctx;
+        std::lock_guard<std::mutex> lk(c->spare_mu);
+        c->spare.keep_larger(b->dev);
+        c->spare_blobs.keep_larger(b->h_blobs);
+        c->spare_results.keep_larger(b->h_results);
+        c->spare_prefix.keep_larger(b->h_prefix);
+        c->spare_packed.keep_larger(b->h_packed);
+    }
     b->dev.release();
     b->h_blobs.release();
     b->h_results.release();
@@ -332,6 +360,14 @@ clf_status clf_batch_create(clf_context* ctx, int32_t n, const uint8_t* const* e
     clf_batch* b = new clf_batch();
     b->ctx = ctx;
     b->n = n;
+    {
+        std::lock_guard<std::mutex> lk(ctx->spare_mu);
+        b->dev.keep_larger(ctx->spare);
+        b->h_blobs.keep_larger(ctx->spare_blobs);
+        b->h_results.keep_larger(ctx->spare_results);
+        b->h_prefix.keep_larger(ctx->spare_prefix);
+        b->h_packed.keep_larger(ctx->spare_packed);
+    }
     b->h_desc.resize(n);
     std::vector<double> cost(n, 0.0);
     size_t off = 0;

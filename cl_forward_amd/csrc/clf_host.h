@@ -59,6 +59,18 @@ struct DevBuf {
     T* as() const {
         return static_cast<T*>(p);
     }
+    // keeps the larger of the two allocations here and frees the other
+    void keep_larger(DevBuf& o) {
+        if (o.cap > cap) {
+            release();
+            p = o.p;
+            cap = o.cap;
+            o.p = nullptr;
+            o.cap = 0;
+        } else {
+            o.release();
+        }
+    }
 };
 
 // Pinned host memory.  `flags` as for hipHostMalloc; hipHostMallocCoherent
@@ -86,6 +98,17 @@ struct PinnedBuf {
     template <class T>
     T* as() const {
         return static_cast<T*>(p);
+    }
+    void keep_larger(PinnedBuf& o) {
+        if (o.cap > cap) {
+            release();
+            p = o.p;
+            cap = o.cap;
+            o.p = nullptr;
+            o.cap = 0;
+        } else {
+            o.release();
+        }
     }
 };
 
@@ -115,6 +138,7 @@ struct Resident {
     // n events, blob_bytes of packed blobs, hits in total, track capacity
     hipError_t ensure(int32_t n, size_t blob_bytes, int64_t hits, int64_t cap);
     void release();
+    void keep_larger(Resident& o);
     SearchArgs args(int32_t n) const;
 };
 

@@ -130,6 +130,9 @@ struct Shared {
     // w2[g & 1] (each buffer lives from its write to its last read)
     W1Rec w1[2][C];
     W2Rec w2[2][C];
+    // which of the h2 windows in w2[k] are empty (bit per hit), and how many
+    uint32_t w2_empty[2][C / 32];
+    int32_t w2_n_empty[2];
     uint32_t used_bits[kUsedBits / 32];
     int32_t scan[3][W];
     int32_t item;
@@ -417,6 +420,10 @@ __device__ __forceinline__ void window_item(Shared<B, C>& sh, const EventView& e
         r.lo = (int16_t)w.x;
         r.hi = (int16_t)w.y;
         w2_of<L>(sh, ev, g2)[i] = r;
+        if (w.x < 0 && (L || sh.s_num[g2] <= C)) {  // rare: the row stops look here
+            atomicOr(&sh.w2_empty[g2 & 1][i >> 5], 1u << (i & 31));
+            atomicAdd(&sh.w2_n_empty[g2 & 1], 1);
+        }
     } else {
         i -= n_w2;
         const SView src = sensor_view<L>(sh, ev, g1);
@@ -508,6 +515,8 @@ struct StepCtx {
     SView v0, v2, v4;  // sensors f, f-2, f-4
     const W1Rec* w1;   // h1 windows + row stops of sensor f
     const W2Rec* w2;   // h2 windows of sensor f-2 (relative to f-4)
+    const uint32_t* w2_marks;   // their empty-window bitmask (nullptr: read w2)
+    const int32_t* w2_n_empty;  // number of marks
     bool fast4;        // sensor f-4 x-sorted and the event in the supported domain
     bool seeds;        // h1 sensor f-2 >= 4 has h2 windows (Kernel.cl:58,104)
 };
@@ -541,6 +550,22 @@ __device__ void seed_batch(Shared<B, C>& sh, const StepCtx& sc, const UsedFlags&
         int16_t d[4] = {kNoStop, kNoStop, kNoStop, kNoStop};
         if (!sc.seeds) {
             d[0] = d[1] = d[2] = d[3] = 0;
+        } else if (sc.w2_marks) {
+            // the empty windows of sensor f-2 are marked in a bitmask: the
+            // first marked element of each row's residue class
+            if (*sc.w2_n_empty > 0 && nh > 0) {
+                const int a0 = r.lo, a1 = r.lo + nh;
+                for (int wi = a0 >> 5; wi <= (a1 - 1) >> 5; ++wi) {
+                    uint32_t m = sc.w2_marks[wi];
+                    if (wi == (a0 >> 5)) m &= ~0u << (a0 & 31);
+                    if (wi == ((a1 - 1) >> 5) && (a1 & 31)) m &= (1u << (a1 & 31)) - 1u;
+                    while (m) {
+                        const int el = wi * 32 + __builtin_ctz(m) - a0;
+                        m &= m - 1u;
+                        if (d[el & 3] == kNoStop) d[el & 3] = (int16_t)(el >> 2);
+                    }
+                }
+            }
         } else {
             const W2Rec* w2 = sc.w2 + r.lo;
             int found = 0;
@@ -706,6 +731,14 @@ __device__ __forceinline__ void stage_sensor(Shared<B, C>& sh, const EventView& 
     }
 }
 
+// Clears the empty-window marks of w2 buffer k (all threads; a barrier must
+// follow before the windows that fill them).
+template <int B, int C>
+__device__ __forceinline__ void clear_empty_marks(Shared<B, C>& sh, int k) {
+    for (int i = threadIdx.x; i < C / 32; i += B) sh.w2_empty[k][i] = 0u;
+    if (threadIdx.x == 0) sh.w2_n_empty[k] = 0;
+}
+
 // Number of window items for the step whose h1 sensor is g2 and h0 sensor g1.
 template <int B, int C>
 __device__ __forceinline__ int n_w2_items(const Shared<B, C>& sh, int g2) {
@@ -744,6 +777,9 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
     sc.v4 = sensor_view<L>(sh, ev, f - 4);
     sc.w1 = w1_of<L>(sh, ev, f);
     sc.w2 = w2_of<L>(sh, ev, f - 2);
+    const bool marks = f - 2 >= 0 && (L || sh.s_num[f - 2] <= C);
+    sc.w2_marks = marks ? sh.w2_empty[(f - 2) & 1] : nullptr;
+    sc.w2_n_empty = &sh.w2_n_empty[(f - 2) & 1];
     sc.fast4 = sc.v4.sorted && !degenerate;
     sc.seeds = f - 2 >= 4;
     const SView& v4 = sc.v4;
@@ -1007,6 +1043,8 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
     }
     // the ring starts with sensors S-1 .. S-5
     for (int s = S - 1; s >= S - 5 && s >= 0; --s) stage_sensor<B, C>(sh, ev, s);
+    clear_empty_marks<B, C>(sh, 0);
+    clear_empty_marks<B, C>(sh, 1);
     __syncthreads();
     const bool degenerate = (sh.flags & CLF_FLAG_DEGENERATE_Z) != 0;
 
@@ -1054,6 +1092,7 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
     int n_cur = 0, n_tracks = 0, n_tracklets = 0, n_weak = 0, visits = 0;
     for (int f = S - 1; f >= 4; --f) {
         stage_sensor<B, C>(sh, ev, f - 5);
+        clear_empty_marks<B, C>(sh, (f - 3) & 1);  // filled by this step's windows of f-3
         __syncthreads();
         prof_mark(sh, pc, PROF_STAGE);
         prof_add(sh, pc, CNT_FWD_ENTRIES, (unsigned long long)n_cur);
