@@ -50,6 +50,7 @@ constexpr int kRing = 6;          // LDS sensor slots: f .. f-5
 constexpr int kUsedBits = 32768;  // events up to this many hits keep used flags in LDS
 constexpr float kLim = 1.0e5f;    // |coordinate| bound of the supported domain
 constexpr int16_t kNoStop = 0x7fff;
+constexpr int kBuckets = 128;     // x-bucket index per LDS sensor slot
 
 // Phase profile slots (wall-clock ticks, then work counters), filled when
 // SearchArgs::prof is set.  Thread 0 takes a timestamp after the barriers
@@ -126,6 +127,10 @@ struct Shared {
     float rx[kRing][C];
     float ry[kRing][C];
     float rz[kRing][C];
+    // x-bucket index of each slot (sorted sensors): the first hit of bucket
+    // >= j, bucket(x) = clamp(floor((x - x0) * scale))
+    float bk_x0[kRing], bk_scale[kRing];
+    int16_t bk_start[kRing][kBuckets + 1];
     // windows: sensor g's h1 windows in w1[g & 1], its h2 windows in
     // w2[g & 1] (each buffer lives from its write to its last read)
     W1Rec w1[2][C];
@@ -235,6 +240,46 @@ __device__ __forceinline__ int first_diff_ge(const float* x, int lo, int hi, flo
     return lo;
 }
 
+// ---------------------------------------------------------------------------
+// The same searches through a slot's x-bucket index.  bucket() is
+// nondecreasing in x (IEEE subtraction and multiplication by scale >= 0 are
+// monotone, and so are the truncation and the clamps; NaN goes to 0), so
+// every hit before start[bucket(v)] has x < v.  A search starts there (or at
+// a bucket whose threshold is below the exact one by a margin, for
+// predicates on a rounded difference) and scans with the exact predicate,
+// four loads in flight; a bucket holds a few hits, so the scan is short.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int bucket_of(float x, float x0, float scale) {
+    const float t = (x - x0) * scale;
+    return t >= (float)(kBuckets - 1) ? kBuckets - 1 : (t >= 0.f ? (int)t : 0);
+}
+// first i in [i, hi) with pred(x[i]) (pred false then true along x)
+template <class P>
+__device__ __forceinline__ int scan_to(const float* x, int i, int hi, P pred) {
+    for (; i < hi; i += 4) {
+        const float a0 = x[i];
+        const float a1 = x[min(i + 1, hi - 1)];
+        const float a2 = x[min(i + 2, hi - 1)];
+        const float a3 = x[min(i + 3, hi - 1)];
+        if (pred(a0)) return i;
+        if (i + 1 < hi && pred(a1)) return i + 1;
+        if (i + 2 < hi && pred(a2)) return i + 2;
+        if (i + 3 < hi && pred(a3)) return i + 3;
+    }
+    return hi;
+}
+// start index for a search in [lo, hi) whose predicate fails below v
+__device__ __forceinline__ int bucket_start(const int16_t* bs, float x0, float scale, float v, int lo,
+                                            int hi) {
+    const int st = bs[bucket_of(v, x0, scale)];
+    return st < lo ? lo : (st > hi ? hi : st);
+}
+// threshold below which fl(x - c) > t (and >= t) fail, with a margin that
+// covers the rounding of the difference and of the threshold itself
+__device__ __forceinline__ float diff_threshold(float c, float t) {
+    return (c + t) - (2.0e-3f + 1.0e-5f * (fabsf(c) + fabsf(t)));
+}
+
 // Widened bounds [lo, hi] of the x of hits that can pass |x_pred - x| < 0.6.
 // x_pred over a sensor is bracketed by pa and pb, its values at the sensor's
 // extreme z (the prediction chain is monotone in z under IEEE rounding).  If
@@ -273,7 +318,30 @@ struct SView {
     int n;
     bool sorted;
     float zmin, zmax;
+    const int16_t* bstart;  // x-bucket index (LDS slot of a sorted sensor), else nullptr
+    float bx0, bscale;
 };
+
+__device__ __forceinline__ int sv_lb_ge(const SView& v, int lo, int hi, float val) {
+    if (!v.bstart) return lb_ge(v.x, lo, hi, val);
+    return scan_to(v.x, bucket_start(v.bstart, v.bx0, v.bscale, val, lo, hi), hi,
+                   [=](float a) { return a >= val; });
+}
+__device__ __forceinline__ int sv_ub_gt(const SView& v, int lo, int hi, float val) {
+    if (!v.bstart) return ub_gt(v.x, lo, hi, val);
+    return scan_to(v.x, bucket_start(v.bstart, v.bx0, v.bscale, val, lo, hi), hi,
+                   [=](float a) { return a > val; });
+}
+__device__ __forceinline__ int sv_first_diff_gt(const SView& v, int lo, int hi, float c, float t) {
+    if (!v.bstart) return first_diff_gt(v.x, lo, hi, c, t);
+    return scan_to(v.x, bucket_start(v.bstart, v.bx0, v.bscale, diff_threshold(c, t), lo, hi), hi,
+                   [=](float a) { return (a - c) > t; });
+}
+__device__ __forceinline__ int sv_first_diff_ge(const SView& v, int lo, int hi, float c, float t) {
+    if (!v.bstart) return first_diff_ge(v.x, lo, hi, c, t);
+    return scan_to(v.x, bucket_start(v.bstart, v.bx0, v.bscale, diff_threshold(c, t), lo, hi), hi,
+                   [=](float a) { return (a - c) >= t; });
+}
 
 struct EventView {
     int S, N;
@@ -295,11 +363,19 @@ __device__ __forceinline__ SView sensor_view(Shared<B, C>& sh, const EventView& 
     v.sorted = sh.s_sorted[s] != 0;
     v.zmin = sh.s_zmin[s];
     v.zmax = sh.s_zmax[s];
+    v.bstart = nullptr;
+    v.bx0 = 0.f;
+    v.bscale = 0.f;
     if (L || v.n <= C) {
         const int k = s % kRing;
         v.x = sh.rx[k];
         v.y = sh.ry[k];
         v.z = sh.rz[k];
+        if (v.sorted) {
+            v.bstart = sh.bk_start[k];
+            v.bx0 = sh.bk_x0[k];
+            v.bscale = sh.bk_scale[k];
+        }
     } else {
         v.x = ev.X + v.start;
         v.y = ev.Y + v.start;
@@ -319,6 +395,9 @@ __device__ __forceinline__ SView hbm_view(Shared<B, C>& sh, const EventView& ev,
     v.x = ev.X + v.start;
     v.y = ev.Y + v.start;
     v.z = ev.Zh + v.start;
+    v.bstart = nullptr;
+    v.bx0 = 0.f;
+    v.bscale = 0.f;
     return v;
 }
 
@@ -345,8 +424,8 @@ __device__ int2 window_h1(const SView& s2, float x0, float z0) {
     if (s2.sorted && s2.zmin == s2.zmax) {
         // one z on the sensor: the predicate is -D < fl(x1-x0) < D
         const float D = CLF_PARAM_MAXXSLOPE_CANDIDATES * fabsf(s2.zmin - z0);
-        const int lo = first_diff_gt(X, 0, n, x0, -D);
-        const int hi = first_diff_ge(X, lo, n, x0, D);
+        const int lo = sv_first_diff_gt(s2, 0, n, x0, -D);
+        const int hi = sv_first_diff_ge(s2, lo, n, x0, D);
         if (lo < hi) w = make_int2(lo, hi);
         return w;
     }
@@ -355,7 +434,7 @@ __device__ int2 window_h1(const SView& s2, float x0, float z0) {
         // no hit with fl(x1-x0) <= -Bd can pass, Bd >= every 0.7|z1-z0|
         const float a = fabsf(s2.zmin - z0), b = fabsf(s2.zmax - z0);
         const float Bd = CLF_PARAM_MAXXSLOPE_CANDIDATES * fmaxf(a, b);
-        i = first_diff_gt(X, 0, n, x0, -Bd);
+        i = sv_first_diff_gt(s2, 0, n, x0, -Bd);
     }
     const float* Z = s2.z;
     for (; i < n; ++i) {
@@ -390,8 +469,8 @@ __device__ int2 window_h2(const SView& s2, float x0, float z0, int zs0, int zs2)
     const float* X = s2.x;
     if (n == 0) return w;
     if (s2.sorted) {
-        const int first = ub_gt(X, 0, n, xmin_h2);
-        if (first < n) w = make_int2(first, ub_gt(X, first + 1, n, xmax_h2));
+        const int first = sv_ub_gt(s2, 0, n, xmin_h2);
+        if (first < n) w = make_int2(first, sv_ub_gt(s2, first + 1, n, xmax_h2));
         return w;
     }
     int i = 0;
@@ -652,7 +731,7 @@ __device__ void seed_batch(Shared<B, C>& sh, const StepCtx& sc, const UsedFlags&
             const float pb = x0 + (x1 - x0) * ((v4.zmax - z0) * dz_inverted);
             float lo;
             x_window(pa, pb, lo, hi);
-            k = lb_ge(v4.x, k, k1, lo);
+            k = sv_lb_ge(v4, k, k1, lo);
         }
         float best = FLT_MAX;
         int bk = -1;
@@ -728,6 +807,24 @@ __device__ __forceinline__ void stage_sensor(Shared<B, C>& sh, const EventView& 
         sh.rx[k][i] = ev.X[b + i];
         sh.ry[k][i] = ev.Y[b + i];
         sh.rz[k][i] = ev.Zh[b + i];
+    }
+    if (!sh.s_sorted[s]) return;
+    // x-bucket index: hit i writes start[j] for the buckets j it opens
+    const float x0 = n > 0 ? ev.X[b] : 0.f;
+    const float span = n > 1 ? ev.X[b + n - 1] - x0 : 0.f;
+    const float scale = span > 0.f ? (float)kBuckets / span : 0.f;
+    if (threadIdx.x == 0) {
+        sh.bk_x0[k] = x0;
+        sh.bk_scale[k] = scale;
+        if (n == 0)
+            for (int j = 0; j <= kBuckets; ++j) sh.bk_start[k][j] = 0;
+    }
+    for (int i = threadIdx.x; i < n; i += B) {
+        const int bi = bucket_of(ev.X[b + i], x0, scale);
+        const int bp = i > 0 ? bucket_of(ev.X[b + i - 1], x0, scale) : -1;
+        for (int j = bp + 1; j <= bi; ++j) sh.bk_start[k][j] = (int16_t)i;
+        if (i == n - 1)
+            for (int j = bi + 1; j <= kBuckets; ++j) sh.bk_start[k][j] = (int16_t)n;
     }
 }
 
@@ -817,7 +914,7 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
                 const float pb = t.x0 + tx * (v4.zmax - t.z0);
                 float lo;
                 x_window(pa, pb, lo, hi);
-                k = lb_ge(v4.x, 0, v4.n, lo);
+                k = sv_lb_ge(v4, 0, v4.n, lo);
             }
             float best_fit = FLT_MAX;
             for (; k < v4.n; ++k) {
