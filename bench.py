@@ -62,7 +62,7 @@ def cpu_baseline(blobs, gpu_outs, threads: int):
 
     oracle.build()
     t0 = time.perf_counter()
-    outs, _ = oracle.search_batch(blobs, n_threads=threads, keep_output=True)
+    outs, ostats = oracle.search_batch(blobs, n_threads=threads, keep_output=True)
     dt = time.perf_counter() - t0
     identical = sum(int(a == b) for a, b in zip(outs, gpu_outs))
     # one core on a bounded sample, for reference
@@ -79,7 +79,7 @@ def cpu_baseline(blobs, gpu_outs, threads: int):
                   f"{threads} threads ({cpu_model()}); 1-core rate on the first {n1}: "
                   f"{round(n1 / dt1, 2)} events/s",
         "value_1core": round(n1 / dt1, 2),
-    }, identical
+    }, identical, ostats
 
 
 def serving_runs(ctx, a):
@@ -313,10 +313,23 @@ def main(argv=None):
 
     cpu = None
     oracle_identical = None
+    valu = None
     if not a.no_cpu_baseline:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads")
-        cpu, oracle_identical = cpu_baseline(blobs, outs, threads)
+        cpu, oracle_identical, ostats = cpu_baseline(blobs, outs, threads)
+        # the reference algorithm's FP32 work, counted by the oracle: about 20
+        # ops per evaluated seeding triplet (h2 in the window of a live pair)
+        # and 15 per forwarding fit (a followed track against every hit of
+        # its target sensor); the kernel skips the triplets and fits its exact
+        # x-ranges rule out, so this is the reference's work, not the kernel's
+        trip = sum(st["seed_triplets"] for st in ostats)
+        fits = sum(st["fwd_fits"] for st in ostats)
+        flop = 20 * trip + 15 * fits
+        tf = flop / (dev["ms_search"] / 1e3) / 1e12
+        valu = {"bound": "valu", "algorithmic_flop": flop, "seed_triplets": trip, "fwd_fits": fits,
+                "achieved": round(tf, 3), "peak": 157.3, "unit": "TFLOP/s",
+                "frac": round(tf / 157.3, 5)}
         log(f"cpu: {cpu['value']} events/s; oracle-identical events {oracle_identical}/{n_events}")
 
     serving = None if a.no_serve else serving_runs(ctx, a)
@@ -354,6 +367,7 @@ def main(argv=None):
         "end_to_end": e2e,
         "oracle_identical_events": oracle_identical,
         "roofline": roofline,
+        "valu_roofline": valu,
         "cpu_baseline": cpu,
         "vs_cpu_baseline": round(value / cpu["value"], 1) if cpu else None,
         "serving_p50_ms_at_50pct": at50["p50_ms"] if at50 else None,

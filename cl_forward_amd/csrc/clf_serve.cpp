@@ -798,7 +798,7 @@ void clf_server_default_config(clf_server_config* cfg) {
     cfg->max_requests = 65536;
     cfg->arena_bytes = (int64_t)512 << 20;
     cfg->out_arena_bytes = (int64_t)1 << 30;
-    cfg->max_event_hits = 16384;
+    cfg->max_event_hits = 32768;
     cfg->idle_us = 5000;
     cfg->slots = 4;
     cfg->max_batch_events = 256;

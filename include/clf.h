@@ -223,7 +223,7 @@ typedef struct clf_served {
 } clf_served;
 
 /* Fills *cfg with the defaults: resident engine, 65536 requests, 512 MiB
- * input and 1 GiB output arenas, events up to 16384 hits, 5000 us idle,
+ * input and 1 GiB output arenas, events up to 32768 hits, 5000 us idle,
  * output regions of 256 + N/2 records; for the batch engine 4 slots of 256
  * events / 2^20 hits, 100 us. */
 void clf_server_default_config(clf_server_config* cfg);

@@ -165,3 +165,16 @@ def test_resident_spill_path_is_exact(gpu_ctx, plain51_blobs, plain_ref):
         rec, summ = srv.replay(plain51_blobs, pick, arrival, submit_threads=2)
     assert summ["completed"] == n
     assert np.array_equal(rec["digest"], want[pick])
+
+
+def test_resident_serves_stress_events(gpu_ctx, oracle_mod):
+    """Events of the stress workload (up to ~25,000 hits, sensors above an
+    LDS slot) through the resident engine: identical to the oracle."""
+    from cl_forward_amd import generator
+
+    blobs = [generator.stress_event(i).pack() for i in (0, 1, 2, 7)]
+    with Server(gpu_ctx, engine=SERVE_RESIDENT) as srv:
+        tickets = [srv.submit(b) for b in blobs]
+        got = [srv.wait(t, timeout_ms=60000)[0] for t in tickets]
+    for b, g in zip(blobs, got):
+        assert g == oracle_mod.search_event(b)[0]
