@@ -36,6 +36,8 @@ class LaunchConfig(ctypes.Structure):
         ("grain", ctypes.c_int32),
         ("use_graph", ctypes.c_int32),
         ("lds_hits", ctypes.c_int32),
+        ("heavy_events", ctypes.c_int32),
+        ("heavy_block", ctypes.c_int32),
     ]
 
 

@@ -2,8 +2,11 @@
 
 Sweeps the workgroup size {64, 128, 256, 512}, the hits per LDS sensor slot
 {256, 512} and the persistent grid (a multiple of the compute-unit count, or
-the occupancy-derived default) on a resident batch, checks that every configuration returns the same bytes,
-and records the fastest in ``cl_forward_amd/tuned.json``:
+the occupancy-derived default) on a resident batch, then, for the best of
+those, the heavy-event lane (the costliest events in a second launch of
+512-thread workgroups: 0, 8, 16, 32, 64 or 128 events); checks that every
+configuration returns the same bytes, and records the fastest in
+``cl_forward_amd/tuned.json``:
 
     python -m cl_forward_amd.autotune [--workload batch1000] [--reps 5]
 
@@ -26,6 +29,7 @@ TUNED_PATH = os.path.join(HERE, "tuned.json")
 BLOCKS = (64, 128, 256, 512)
 LDS_HITS = (256, 512)
 GRID_PER_CU = (0, 1, 2, 4)  # 0 = occupancy-derived default
+HEAVY_EVENTS = (0, 8, 16, 32, 64, 128)
 
 
 def load_tuned(num_cus: int, workload: str | None = None) -> dict | None:
@@ -46,27 +50,34 @@ def tune(ctx, batch, reps: int = 5, warmup: int = 1, log=print) -> dict:
     from .search import MODE_BATCHED
 
     table = []
-    ref_digest = None
-    for block, lds_hits, per_cu in itertools.product(BLOCKS, LDS_HITS, GRID_PER_CU):
-        grid = per_cu * ctx.num_cus
-        ctx.set_config(block_size=block, grid_size=grid, use_graph=1, lds_hits=lds_hits)
+    ref = {}
+
+    def measure(**cfg):
+        ctx.set_config(use_graph=1, **cfg)
         t = batch.time(MODE_BATCHED, warmup=warmup, reps=reps)
         batch.run(MODE_BATCHED)
         outs, _ = batch.results()
         digest = hashlib.sha256(b"".join(outs)).hexdigest()
-        if ref_digest is None:
-            ref_digest = digest
-        same = digest == ref_digest
-        row = {"block_size": block, "lds_hits": lds_hits, "grid_size": grid,
-               "ms_run": round(t["ms_run"], 4), "ms_search": round(t["ms_search"], 4),
-               "identical": same}
+        ref.setdefault("digest", digest)
+        row = dict(cfg, ms_run=round(t["ms_run"], 4), ms_search=round(t["ms_search"], 4),
+                   identical=digest == ref["digest"])
         table.append(row)
         log(json.dumps(row), flush=True) if log is print else log(row)
-        if not same:
+        if not row["identical"]:
             raise RuntimeError(f"launch configuration {row} changed the results")
+        return row
+
+    for block, lds_hits, per_cu in itertools.product(BLOCKS, LDS_HITS, GRID_PER_CU):
+        measure(block_size=block, grid_size=per_cu * ctx.num_cus, lds_hits=lds_hits,
+                heavy_events=0, heavy_block=512)
+    base = min(table, key=lambda r: r["ms_run"])
+    for k in HEAVY_EVENTS[1:]:
+        measure(block_size=base["block_size"], grid_size=base["grid_size"],
+                lds_hits=base["lds_hits"], heavy_events=k, heavy_block=512)
     best = min(table, key=lambda r: r["ms_run"])
     return {"best": {"block_size": best["block_size"], "grid_size": best["grid_size"],
-                     "lds_hits": best["lds_hits"], "grain": 1, "use_graph": 1},
+                     "lds_hits": best["lds_hits"], "grain": 1, "use_graph": 1,
+                     "heavy_events": best["heavy_events"], "heavy_block": best["heavy_block"]},
             "table": table}
 
 

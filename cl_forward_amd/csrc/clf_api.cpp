@@ -121,13 +121,33 @@ int resolved_grid(const clf_launch_config& cfg, int n_cu, int n_items) {
 
 clf_status enqueue_search(const Resident& r, int32_t n, clf_mode mode,
                           const clf_launch_config& cfg, int n_cu, hipStream_t st, hipEvent_t t0,
-                          hipEvent_t t1) {
+                          hipEvent_t t1, hipStream_t side, hipEvent_t fork, hipEvent_t join) {
     if (n == 0) return CLF_OK;
     // queue counters: one per launch (n+1 so plain mode has one per event)
     HIP_TRY(launch_zero_i32(r.queue.as<int32_t>(), n + 1, st));
     SearchArgs a = r.args(n);
     if (t0) HIP_TRY(hipEventRecord(t0, st));
-    if (mode == CLF_MODE_BATCHED) {
+    const int heavy = side && fork && join && cfg.heavy_events > 0 ? std::min(cfg.heavy_events, n) : 0;
+    if (mode == CLF_MODE_BATCHED && heavy > 0) {
+        // the costliest events (the front of the cost order) in a launch of
+        // larger workgroups, one per event, on the side stream; the rest in
+        // the main launch; joined before packing
+        HIP_TRY(hipEventRecord(fork, st));
+        HIP_TRY(hipStreamWaitEvent(side, fork, 0));
+        SearchArgs h = a;
+        h.first_item = 0;
+        h.n_items = heavy;
+        h.queue_head = r.queue.as<int32_t>() + 1;
+        HIP_TRY(launch_search(h, cfg.heavy_block, cfg.lds_hits, heavy, side));
+        HIP_TRY(hipEventRecord(join, side));
+        if (heavy < n) {
+            a.first_item = heavy;
+            a.n_items = n - heavy;
+            HIP_TRY(launch_search(a, cfg.block_size, cfg.lds_hits,
+                                  resolved_grid(cfg, n_cu, n - heavy), st));
+        }
+        HIP_TRY(hipStreamWaitEvent(st, join, 0));
+    } else if (mode == CLF_MODE_BATCHED) {
         HIP_TRY(launch_search(a, cfg.block_size, cfg.lds_hits, resolved_grid(cfg, n_cu, n), st));
     } else {
         // plain: event i alone, in its own launch of one workgroup
@@ -153,7 +173,9 @@ struct clf_context {
     char name[256] = {0};
     hipStream_t stream = nullptr;
     int n_cu = 0;
-    clf_launch_config cfg = {256, 0, 1, 1, 256};
+    clf_launch_config cfg = {256, 0, 1, 1, 256, 0, 512};
+    hipStream_t side = nullptr;  // the heavy-event launch of the batched mode
+    hipEvent_t fork = nullptr, join = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     // buffers of destroyed batches, reused by the next batch of this context
     // (the one-call clf_search then allocates nothing in steady state)
@@ -188,12 +210,14 @@ hipError_t current_device(clf_context* ctx) { return hipSetDevice(ctx->device); 
 // The kernels of one run, enqueued on the context stream.
 clf_status enqueue_run(clf_context* ctx, clf_batch* b, clf_mode mode, bool time_search) {
     return enqueue_search(b->dev, b->n, mode, ctx->cfg, ctx->n_cu, ctx->stream,
-                          time_search ? ctx->ev2 : nullptr, time_search ? ctx->ev3 : nullptr);
+                          time_search ? ctx->ev2 : nullptr, time_search ? ctx->ev3 : nullptr,
+                          ctx->side, ctx->fork, ctx->join);
 }
 
 bool same_cfg(const clf_launch_config& x, const clf_launch_config& y) {
     return x.block_size == y.block_size && x.grid_size == y.grid_size && x.grain == y.grain &&
-           x.lds_hits == y.lds_hits;
+           x.lds_hits == y.lds_hits && x.heavy_events == y.heavy_events &&
+           x.heavy_block == y.heavy_block;
 }
 
 clf_status run(clf_context* ctx, clf_batch* b, clf_mode mode) {
@@ -268,6 +292,9 @@ clf_status clf_context_create(int device, clf_context** out) {
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev2);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev3);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming);
     if (e != hipSuccess) {
         clf_context_destroy(ctx);
         return hip_fail(e, "stream/event creation");
@@ -284,6 +311,10 @@ void clf_context_destroy(clf_context* ctx) {
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
     if (ctx->ev3) (void)hipEventDestroy(ctx->ev3);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
+    if (ctx->fork) (void)hipEventDestroy(ctx->fork);
+    if (ctx->join) (void)hipEventDestroy(ctx->join);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     ctx->spare.release();
     ctx->spare_blobs.release();
@@ -300,6 +331,9 @@ clf_status clf_context_set_config(clf_context* ctx, const clf_launch_config* cfg
                     "unsupported (block_size, lds_hits): block 64/128/256/512, lds_hits 256/512");
     if (cfg->grid_size < 0) return fail(CLF_E_INVALID_ARGUMENT, "grid_size must be >= 0");
     if (cfg->grain != 1) return fail(CLF_E_INVALID_ARGUMENT, "grain must be 1");
+    if (cfg->heavy_events < 0 ||
+        (cfg->heavy_events > 0 && !config_supported(cfg->heavy_block, cfg->lds_hits)))
+        return fail(CLF_E_INVALID_ARGUMENT, "heavy_events must be >= 0 with a supported heavy_block");
     ctx->cfg = *cfg;
     return CLF_OK;
 }

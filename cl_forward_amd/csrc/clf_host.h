@@ -151,8 +151,11 @@ int resolved_grid(const clf_launch_config& cfg, int n_cu, int n_items);
 // Enqueues one search of n resident events on `stream`: queue reset, the
 // search kernel(s), track prefix and packing.  t0/t1 (nullable) are
 // recorded right around the search kernel(s).
+// side/fork/join (nullable): the stream and events of the heavy-event
+// launch (cfg.heavy_events); without them everything runs in one launch.
 clf_status enqueue_search(const Resident& r, int32_t n, clf_mode mode,
                           const clf_launch_config& cfg, int n_cu, hipStream_t stream,
-                          hipEvent_t t0, hipEvent_t t1);
+                          hipEvent_t t0, hipEvent_t t1, hipStream_t side = nullptr,
+                          hipEvent_t fork = nullptr, hipEvent_t join = nullptr);
 
 }  // namespace clf

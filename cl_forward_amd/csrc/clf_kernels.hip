@@ -858,7 +858,8 @@ struct StepState {
 template <bool L, int B, int C>
 __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
                                          const UsedFlags& used, clf_track* tracks, uint4* weak,
-                                         bool degenerate, int f, StepState& st, PhaseClock& pc) {
+                                         bool degenerate, int f, StepState& st, PhaseClock& pc,
+                                         const TtfEntry& first) {
     const int tid = threadIdx.x, wave = wave_id();
     int& n_cur = st.n_cur;
     int& n_tracks = st.n_tracks;
@@ -899,7 +900,7 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
         bool found = false, f_new = false, f_req = false, f_weak = false;
         int bk = -1;
         if (active) {
-            t = cur[i];
+            t = base == 0 ? first : cur[i];  // the first tile was loaded ahead
             const bool is_tl = (t.word & CLF_TTF_TRACKLET_BIT) != 0;
             const uint32_t skipped = (t.word & CLF_TTF_SKIP_MASK) >> CLF_TTF_SKIP_SHIFT;
             const float td = 1.0f / (t.z1 - t.z0);
@@ -1187,6 +1188,9 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
     prof_mark(sh, pc, PROF_PROLOGUE);
 
     int n_cur = 0, n_tracks = 0, n_tracklets = 0, n_weak = 0, visits = 0;
+    // each thread's entry of the next forwarding tile, loaded at the end of a
+    // step so that its latency hides behind the staging of the next one
+    TtfEntry first = {};
     for (int f = S - 1; f >= 4; --f) {
         stage_sensor<B, C>(sh, ev, f - 5);
         clear_empty_marks<B, C>(sh, (f - 3) & 1);  // filled by this step's windows of f-3
@@ -1198,9 +1202,9 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
         bool lds_step = true;
         for (int s2 = f; s2 >= f - 5 && s2 >= 0; --s2) lds_step = lds_step && sh.s_num[s2] <= C;
         if (lds_step)
-            run_step<true>(sh, ev, used, tracks, weak, degenerate, f, st, pc);
+            run_step<true>(sh, ev, used, tracks, weak, degenerate, f, st, pc, first);
         else
-            run_step<false>(sh, ev, used, tracks, weak, degenerate, f, st, pc);
+            run_step<false>(sh, ev, used, tracks, weak, degenerate, f, st, pc, first);
         n_cur = st.n_cur;
         n_tracks = st.n_tracks;
         n_tracklets = st.n_tracklets;
@@ -1208,6 +1212,7 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
         visits = st.visits;
         cur = st.cur;
         nxt = st.nxt;
+        if (tid < n_cur) first = cur[tid];
     }
 
     // tail: tracklet entries of the last step become weak (Kernel.cl:771-792)

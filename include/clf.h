@@ -66,6 +66,10 @@ typedef struct clf_launch_config {
     int32_t use_graph;  /* batched mode: capture and replay a hipGraph       */
     int32_t lds_hits;   /* hits per LDS sensor slot, 256 or 512; a sensor
                            with more hits is read from HBM                   */
+    int32_t heavy_events; /* batched mode: the heavy_events costliest events
+                           run in a second launch of heavy_block threads per
+                           event, beside the main launch (0 = one launch)    */
+    int32_t heavy_block;  /* 256 or 512                                     */
 } clf_launch_config;
 
 /* Results of a batch; memory owned by the batch, valid until the batch

@@ -105,6 +105,24 @@ def test_launch_configs_identical(oracle_mod, config_blobs, block, lds_hits):
         _check_search(ctx, oracle_mod, config_blobs, f"block{block}/lds{lds_hits}")
 
 
+@pytest.mark.parametrize("heavy,graph", [(1, 1), (3, 1), (4, 0), (64, 1)])
+def test_heavy_lane_identical(oracle_mod, config_blobs, heavy, graph):
+    """The costliest events in a second launch of 512-thread workgroups
+    (more than the batch holds: all of them), graph-captured or not."""
+    from cl_forward_amd.search import Context
+
+    _say(f"search: heavy lane of {heavy} events, graph {graph}")
+    with Context(0) as ctx:
+        ctx.set_config(block_size=256, lds_hits=512, heavy_events=heavy, heavy_block=512,
+                       use_graph=graph)
+        _check_search(ctx, oracle_mod, config_blobs, f"heavy{heavy}")
+        with ctx.batch(config_blobs) as b:
+            for _ in range(3):
+                b.run(MODE_BATCHED)
+            again, _ = b.results()
+        assert again == [oracle_mod.search_event(x)[0] for x in config_blobs]
+
+
 def test_determinism_and_graph_replays(gpu_ctx, plain51_blobs):
     _say("search: repeated graph replays give identical bytes")
     with gpu_ctx.batch(plain51_blobs) as b:
