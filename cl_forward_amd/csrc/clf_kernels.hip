@@ -139,7 +139,7 @@ struct Shared {
     uint32_t w2_empty[2][C / 32];
     int32_t w2_n_empty[2];
     uint32_t used_bits[kUsedBits / 32];
-    int32_t scan[3][W];
+    int32_t scan[2][3][W];  // two buffers, used in turn by block_scan3
     int32_t item;
     uint32_t flags;
     int32_t max_live;
@@ -176,30 +176,34 @@ __device__ __forceinline__ void prof_add(Shared<B, C>& sh, const PhaseClock& pc,
 }
 
 // Exclusive prefix counts of up to three predicates over the block, in
-// thread order.  Every thread of the block must call it.
+// thread order.  Every thread of the block must call it, with the same
+// `sel` (flipped here).  The per-wave counts alternate between two buffers:
+// a wave can only write a buffer again after the next call's barrier, which
+// every wave reaches after its reads of this call, so one barrier suffices.
 template <int B, int C>
 __device__ __forceinline__ void block_scan3(Shared<B, C>& sh, bool p0, bool p1, bool p2,
-                                            int (&pos)[3], int (&tot)[3]) {
+                                            int (&pos)[3], int (&tot)[3], int& sel) {
     const int lane = lane_id(), wave = wave_id();
     const uint64_t m[3] = {ballot(p0), ballot(p1), ballot(p2)};
+    auto& cnt = sh.scan[sel];
+    sel ^= 1;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         pos[k] = mbcnt(m[k]);
-        if (lane == 0) sh.scan[k][wave] = __popcll(m[k]);
+        if (lane == 0) cnt[k][wave] = __popcll(m[k]);
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         int base = 0, t = 0;
         for (int w = 0; w < Shared<B, C>::W; ++w) {
-            const int c = sh.scan[k][w];
+            const int c = cnt[k][w];
             base += (w < wave) ? c : 0;
             t += c;
         }
         pos[k] += base;
         tot[k] = t;
     }
-    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -609,7 +613,8 @@ struct StepCtx {
 // ---------------------------------------------------------------------------
 template <int B, int C>
 __device__ void seed_batch(Shared<B, C>& sh, const StepCtx& sc, const UsedFlags& used,
-                           TtfEntry* nxt, int nv, int& n_next, int& n_tracklets, PhaseClock& pc) {
+                           TtfEntry* nxt, int nv, int& n_next, int& n_tracklets, PhaseClock& pc,
+                           int& sel) {
     const int tid = threadIdx.x;
     const SView& v0 = sc.v0;
     const SView& v2 = sc.v2;
@@ -768,7 +773,7 @@ __device__ void seed_batch(Shared<B, C>& sh, const StepCtx& sc, const UsedFlags&
         const int vv = base + tid;
         const bool acc = vv < nv && sh.v_best[vv] != ~0ull;
         int pos[3], tot[3];
-        block_scan3<B, C>(sh, acc, false, false, pos, tot);
+        block_scan3<B, C>(sh, acc, false, false, pos, tot, sel);
         if (acc) {
             const unsigned long long key = sh.v_best[vv];
             const int y = (int)((key >> 30) & 3ull);
@@ -849,6 +854,7 @@ __device__ __forceinline__ int n_w1_items(const Shared<B, C>& sh, int g1) {
 // Sensor-loop state of one event (Kernel.cl:669-767).
 struct StepState {
     int n_cur, n_tracks, n_tracklets, n_weak, visits;
+    int sel;        // block_scan3 buffer in turn
     TtfEntry* cur;  // entries inserted in the previous step
     TtfEntry* nxt;  // entries inserted in this step
 };
@@ -943,7 +949,7 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
             }
         }
         int pos[3], tot[3];
-        block_scan3<B, C>(sh, f_new, f_req, f_weak, pos, tot);
+        block_scan3<B, C>(sh, f_new, f_req, f_weak, pos, tot, st.sel);
         if (active) {
             if (found) {
                 const int b = v4.start + bk;
@@ -1017,7 +1023,7 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
         visits += nv;
         prof_add(sh, pc, CNT_VISITS, (unsigned long long)nv);
         prof_add(sh, pc, CNT_BATCHES, 1ull);
-        if (nv > 0) seed_batch<B, C>(sh, sc, used, nxt, nv, n_next, n_tracklets, pc);
+        if (nv > 0) seed_batch<B, C>(sh, sc, used, nxt, nv, n_next, n_tracklets, pc, st.sel);
         __syncthreads();
         prof_mark(sh, pc, PROF_SEED_EMIT);
         if (!more) break;
@@ -1191,6 +1197,7 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
     // each thread's entry of the next forwarding tile, loaded at the end of a
     // step so that its latency hides behind the staging of the next one
     TtfEntry first = {};
+    int sel = 0;
     for (int f = S - 1; f >= 4; --f) {
         stage_sensor<B, C>(sh, ev, f - 5);
         clear_empty_marks<B, C>(sh, (f - 3) & 1);  // filled by this step's windows of f-3
@@ -1198,7 +1205,7 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
         prof_mark(sh, pc, PROF_STAGE);
         prof_add(sh, pc, CNT_FWD_ENTRIES, (unsigned long long)n_cur);
         prof_add(sh, pc, CNT_FWD_TILES, (unsigned long long)((n_cur + B - 1) / B));
-        StepState st = {n_cur, n_tracks, n_tracklets, n_weak, visits, cur, nxt};
+        StepState st = {n_cur, n_tracks, n_tracklets, n_weak, visits, sel, cur, nxt};
         bool lds_step = true;
         for (int s2 = f; s2 >= f - 5 && s2 >= 0; --s2) lds_step = lds_step && sh.s_num[s2] <= C;
         if (lds_step)
@@ -1210,6 +1217,7 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
         n_tracklets = st.n_tracklets;
         n_weak = st.n_weak;
         visits = st.visits;
+        sel = st.sel;
         cur = st.cur;
         nxt = st.nxt;
         if (tid < n_cur) first = cur[tid];
@@ -1221,7 +1229,7 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
         TtfEntry t = {};
         const bool tl = i < n_cur && ((t = cur[i]).word & CLF_TTF_TRACKLET_BIT) != 0;
         int pos[3], tot[3];
-        block_scan3<B, C>(sh, tl, false, false, pos, tot);
+        block_scan3<B, C>(sh, tl, false, false, pos, tot, sel);
         if (tl) weak[n_weak + pos[0]] = make_uint4(t.t0, t.t1, t.t2, 0u);
         n_weak += tot[0];
     }
@@ -1237,7 +1245,7 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
             ok = !used.get((int)tl.x) && !used.get((int)tl.y) && !used.get((int)tl.z);
         }
         int pos[3], tot[3];
-        block_scan3<B, C>(sh, ok, false, false, pos, tot);
+        block_scan3<B, C>(sh, ok, false, false, pos, tot, sel);
         if (ok) {
             uint32_t* r = reinterpret_cast<uint32_t*>(tracks + n_tracks + pos[0]);
             r[0] = 3;
