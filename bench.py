@@ -232,13 +232,13 @@ def main(argv=None):
             phases = {k: {"mean_us": round(float(np.mean(p[k])) / 1e3, 2),
                           "share": round(float(np.sum(p[k]) / max(np.sum(tot), 1)), 4)}
                       for k in batch.PROFILE_FIELDS[:9]}
-            counts = {k: round(float(np.mean(p[k])), 1) for k in batch.PROFILE_FIELDS[10:15]}
+            counts = {k: round(float(np.mean(p[k])), 1) for k in batch.PROFILE_FIELDS[10:16]}
             slow = int(np.argmax(tot))
             report[name] = {"event_us_mean": round(float(np.mean(tot)) / 1e3, 2),
                             "event_us_max": round(float(np.max(tot)) / 1e3, 2),
                             "phases": phases, "counts_mean": counts,
                             "slowest_event": {k: round(float(p[k][slow]) / (1e3 if i < 10 else 1), 2)
-                                              for i, k in enumerate(batch.PROFILE_FIELDS[:15])}}
+                                              for i, k in enumerate(batch.PROFILE_FIELDS[:16])}}
             log(f"{name}: {json.dumps(report[name])}")
         print(json.dumps(report), flush=True)
         return 0

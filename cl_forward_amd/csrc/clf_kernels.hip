@@ -71,6 +71,7 @@ enum ProfSlot {
     CNT_PAIRS = 12,
     CNT_BATCHES = 13,
     CNT_FWD_TILES = 14,
+    CNT_H2_TESTS = 15,    // h2 candidates fitted by the seeding
 };
 
 struct PhaseClock {
@@ -102,6 +103,13 @@ __device__ __forceinline__ int kth_set_bit(uint64_t m, int k) {
         }
     }
     return pos;
+}
+// Values every lane of the wave holds alike (LDS words read at a uniform
+// address, block totals), moved to scalar registers so that they take no
+// vector registers.  Only for values that are the same in all active lanes.
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float uni(float x) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
 }
 __device__ __forceinline__ int wave_incl_scan(int x) {
     const int lane = lane_id();
@@ -202,7 +210,7 @@ __device__ __forceinline__ void block_scan3(Shared<B, C>& sh, bool p0, bool p1, 
             t += c;
         }
         pos[k] += base;
-        tot[k] = t;
+        tot[k] = uni(t);
     }
 }
 
@@ -362,11 +370,11 @@ struct EventView {
 template <bool L, int B, int C>
 __device__ __forceinline__ SView sensor_view(Shared<B, C>& sh, const EventView& ev, int s) {
     SView v;
-    v.start = sh.s_start[s];
-    v.n = sh.s_num[s];
-    v.sorted = sh.s_sorted[s] != 0;
-    v.zmin = sh.s_zmin[s];
-    v.zmax = sh.s_zmax[s];
+    v.start = uni(sh.s_start[s]);
+    v.n = uni(sh.s_num[s]);
+    v.sorted = uni((int)sh.s_sorted[s]) != 0;
+    v.zmin = uni(sh.s_zmin[s]);
+    v.zmax = uni(sh.s_zmax[s]);
     v.bstart = nullptr;
     v.bx0 = 0.f;
     v.bscale = 0.f;
@@ -377,8 +385,8 @@ __device__ __forceinline__ SView sensor_view(Shared<B, C>& sh, const EventView& 
         v.z = sh.rz[k];
         if (v.sorted) {
             v.bstart = sh.bk_start[k];
-            v.bx0 = sh.bk_x0[k];
-            v.bscale = sh.bk_scale[k];
+            v.bx0 = uni(sh.bk_x0[k]);
+            v.bscale = uni(sh.bk_scale[k]);
         }
     } else {
         v.x = ev.X + v.start;
@@ -391,11 +399,11 @@ __device__ __forceinline__ SView sensor_view(Shared<B, C>& sh, const EventView& 
 template <int B, int C>
 __device__ __forceinline__ SView hbm_view(Shared<B, C>& sh, const EventView& ev, int s) {
     SView v;
-    v.start = sh.s_start[s];
-    v.n = sh.s_num[s];
-    v.sorted = sh.s_sorted[s] != 0;
-    v.zmin = sh.s_zmin[s];
-    v.zmax = sh.s_zmax[s];
+    v.start = uni(sh.s_start[s]);
+    v.n = uni(sh.s_num[s]);
+    v.sorted = uni((int)sh.s_sorted[s]) != 0;
+    v.zmin = uni(sh.s_zmin[s]);
+    v.zmax = uni(sh.s_zmax[s]);
     v.x = ev.X + v.start;
     v.y = ev.Y + v.start;
     v.z = ev.Zh + v.start;
@@ -407,11 +415,11 @@ __device__ __forceinline__ SView hbm_view(Shared<B, C>& sh, const EventView& ev,
 
 template <bool L, int B, int C>
 __device__ __forceinline__ W1Rec* w1_of(Shared<B, C>& sh, const EventView& ev, int s) {
-    return (L || sh.s_num[s] <= C) ? sh.w1[s & 1] : ev.hbm_w1 + sh.s_start[s];
+    return (L || uni(sh.s_num[s]) <= C) ? sh.w1[s & 1] : ev.hbm_w1 + uni(sh.s_start[s]);
 }
 template <bool L, int B, int C>
 __device__ __forceinline__ W2Rec* w2_of(Shared<B, C>& sh, const EventView& ev, int s) {
-    return (L || sh.s_num[s] <= C) ? sh.w2[s & 1] : ev.hbm_w2 + sh.s_start[s];
+    return (L || uni(sh.s_num[s]) <= C) ? sh.w2[s & 1] : ev.hbm_w2 + uni(sh.s_start[s]);
 }
 
 // ---------------------------------------------------------------------------
@@ -715,6 +723,7 @@ __device__ void seed_batch(Shared<B, C>& sh, const StepCtx& sc, const UsedFlags&
         }
         v = lo;
     }
+    unsigned n_h2 = 0;
     for (int p = tid; p < P; p += B) {
         while (sh.v_pre[v + 1] <= p) ++v;
         const int e = p - sh.v_pre[v];
@@ -743,6 +752,7 @@ __device__ void seed_batch(Shared<B, C>& sh, const StepCtx& sc, const UsedFlags&
         for (; k < k1; ++k) {
             const float x2 = v4.x[k], y2 = v4.y[k], z2 = v4.z[k];
             if (x2 > hi) break;  // sorted: nothing further can pass
+            if (pc.on) ++n_h2;
             const float z2_tz = (z2 - z0) * dz_inverted;
             const float x = x0 + (x1 - x0) * z2_tz;
             const float yy = y0 + (y1 - y0) * z2_tz;
@@ -765,6 +775,7 @@ __device__ void seed_batch(Shared<B, C>& sh, const StepCtx& sc, const UsedFlags&
             atomicMin(&sh.v_best[v], key);
         }
     }
+    if (pc.on && n_h2) atomicAdd(&sh.prof[CNT_H2_TESTS], (unsigned long long)n_h2);
     __syncthreads();
     prof_mark(sh, pc, PROF_SEED_EVAL);
 
@@ -805,9 +816,9 @@ __device__ void seed_batch(Shared<B, C>& sh, const StepCtx& sc, const UsedFlags&
 template <int B, int C>
 __device__ __forceinline__ void stage_sensor(Shared<B, C>& sh, const EventView& ev, int s) {
     if (s < 0) return;
-    const int n = sh.s_num[s];
+    const int n = uni(sh.s_num[s]);
     if (n > C) return;
-    const int b = sh.s_start[s], k = s % kRing;
+    const int b = uni(sh.s_start[s]), k = s % kRing;
     for (int i = threadIdx.x; i < n; i += B) {
         sh.rx[k][i] = ev.X[b + i];
         sh.ry[k][i] = ev.Y[b + i];
@@ -844,11 +855,11 @@ __device__ __forceinline__ void clear_empty_marks(Shared<B, C>& sh, int k) {
 // Number of window items for the step whose h1 sensor is g2 and h0 sensor g1.
 template <int B, int C>
 __device__ __forceinline__ int n_w2_items(const Shared<B, C>& sh, int g2) {
-    return g2 >= 4 ? sh.s_num[g2] : 0;
+    return g2 >= 4 ? uni(sh.s_num[g2]) : 0;
 }
 template <int B, int C>
 __device__ __forceinline__ int n_w1_items(const Shared<B, C>& sh, int g1) {
-    return g1 >= 4 ? sh.s_num[g1] : 0;
+    return g1 >= 4 ? uni(sh.s_num[g1]) : 0;
 }
 
 // Sensor-loop state of one event (Kernel.cl:669-767).
@@ -1282,13 +1293,26 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
     }
 }
 
+// Waves per SIMD the register allocation is held to (512 registers per
+// lane / waves).  An event's search time is set by its waves' latency
+// chains, so a CU runs more events at once, and a heavy event finishes
+// sooner in a wider workgroup, when more waves fit: four per SIMD (128
+// registers) lets a CU hold four 256-thread or two 512-thread workgroups,
+// which their LDS also allows with 256-hit slots, and two 512-thread
+// workgroups with 512-hit slots.
 template <int B, int C>
-__global__ __launch_bounds__(B) void search_kernel(SearchArgs a) {
+struct MinWaves {
+    static constexpr int value = ((B >= 256 && C == 256) || (B == 512 && C == 512)) ? 4 : 1;
+};
+
+template <int B, int C>
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(MinWaves<B, C>::value)))
+void search_kernel(SearchArgs a) {
     __shared__ Shared<B, C> sh;
     for (;;) {
         if (threadIdx.x == 0) sh.item = atomicAdd(a.queue_head, 1);
         __syncthreads();
-        const int item = sh.item;
+        const int item = uni(sh.item);
         __syncthreads();
         if (item >= a.n_items) break;  // every wave of the group leaves together
         const int idx = a.first_item + item;
@@ -1527,7 +1551,7 @@ __global__ __launch_bounds__(B) void serve_kernel(ServeArgs a) {
             sh.sv_d = d;
         }
         __syncthreads();
-        const int got = sh.item;
+        const int got = uni(sh.item);
         const uint64_t t_claim = sh.prof[0];
         __syncthreads();
         if (got < 0) break;

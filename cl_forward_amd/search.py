@@ -199,7 +199,7 @@ class Batch:
 
     PROFILE_FIELDS = ("prologue", "stage", "forward", "gather", "seed_setup", "seed_stop",
                       "seed_eval", "seed_emit", "tail", "total", "fwd_entries", "visits", "pairs",
-                      "batches", "fwd_tiles", "unused")
+                      "batches", "fwd_tiles", "h2_tests")
 
     def profile(self, mode: int = MODE_BATCHED):
         """Per-event phase times (ns) and work counters of one run, as a dict of
