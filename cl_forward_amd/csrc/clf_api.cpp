@@ -328,7 +328,7 @@ clf_status clf_context_set_config(clf_context* ctx, const clf_launch_config* cfg
     if (!ctx || !cfg) return fail(CLF_E_INVALID_ARGUMENT, "null argument");
     if (!config_supported(cfg->block_size, cfg->lds_hits))
         return fail(CLF_E_INVALID_ARGUMENT,
-                    "unsupported (block_size, lds_hits): block 64/128/256/512, lds_hits 256/512");
+                    "unsupported (block_size, lds_hits): block 64/128/256/512/1024, lds_hits 256/512");
     if (cfg->grid_size < 0) return fail(CLF_E_INVALID_ARGUMENT, "grid_size must be >= 0");
     if (cfg->grain != 1) return fail(CLF_E_INVALID_ARGUMENT, "grain must be 1");
     if (cfg->heavy_events < 0 ||

@@ -1625,10 +1625,12 @@ __global__ __launch_bounds__(256) void probe_copy_kernel(const uint4* __restrict
     X(128, 256)                \
     X(256, 256)                \
     X(512, 256)                \
+    X(1024, 256)               \
     X(64, 512)                 \
     X(128, 512)                \
     X(256, 512)                \
-    X(512, 512)
+    X(512, 512)                \
+    X(1024, 512)
 
 bool config_supported(int block, int lds_hits) {
 #define CLF_CASE(B_, C_) \

@@ -60,7 +60,7 @@ typedef struct clf_event_stats {
 
 /* Launch parameters (no effect on results). */
 typedef struct clf_launch_config {
-    int32_t block_size; /* 64, 128, 256 or 512 threads per event workgroup   */
+    int32_t block_size; /* 64, 128, 256, 512 or 1024 threads per event workgroup */
     int32_t grid_size;  /* persistent workgroups; 0 = CUs x occupancy        */
     int32_t grain;      /* events per dequeue (1)                            */
     int32_t use_graph;  /* batched mode: capture and replay a hipGraph       */
@@ -69,7 +69,7 @@ typedef struct clf_launch_config {
     int32_t heavy_events; /* batched mode: the heavy_events costliest events
                            run in a second launch of heavy_block threads per
                            event, beside the main launch (0 = one launch)    */
-    int32_t heavy_block;  /* 256 or 512                                     */
+    int32_t heavy_block;  /* any supported block_size                       */
 } clf_launch_config;
 
 /* Results of a batch; memory owned by the batch, valid until the batch

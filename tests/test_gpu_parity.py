@@ -94,7 +94,7 @@ def config_blobs(plain51_blobs, mcdata_sorted):
     return blobs
 
 
-@pytest.mark.parametrize("block", [64, 128, 256, 512])
+@pytest.mark.parametrize("block", [64, 128, 256, 512, 1024])
 @pytest.mark.parametrize("lds_hits", [256, 512])
 def test_launch_configs_identical(oracle_mod, config_blobs, block, lds_hits):
     from cl_forward_amd.search import Context

@@ -276,3 +276,18 @@ def test_tracks_digest_matches_restatement(lib):
     for n in list(range(0, 18)) + [37, 100]:
         data = rng.integers(0, 256, 100 * n, dtype=np.uint8).tobytes()
         assert tracks_digest(data) == _digest_py(data, n), n
+
+
+def test_autotune_keeps_single_launch_within_margin():
+    from cl_forward_amd.autotune import HEAVY_MARGIN, pick_best
+
+    def row(ms, heavy=0, block=256):
+        return {"block_size": block, "grid_size": 0, "lds_hits": 256, "heavy_events": heavy,
+                "heavy_block": 512, "ms_run": ms}
+
+    single = row(1.00)
+    near = row(1.00 * (1.0 - HEAVY_MARGIN / 2), heavy=8)
+    far = row(1.00 * (1.0 - 2 * HEAVY_MARGIN), heavy=16)
+    assert pick_best([single, near]) is single
+    assert pick_best([single, near, far]) is far
+    assert pick_best([row(2.0), single, row(1.5, block=512)]) is single
