@@ -122,7 +122,7 @@ __device__ __forceinline__ int wave_incl_scan(int x) {
 }
 
 template <int B, int C>
-struct Shared {
+struct SharedCore {
     static constexpr int W = B / kWave;
     // per-sensor data of the current event
     int32_t s_start[CLF_MAX_SENSORS];
@@ -167,6 +167,30 @@ struct Shared {
     uint64_t sv_in, sv_out;
     int32_t sv_n, sv_s, sv_req, sv_cap;
     uint32_t sv_d;
+};
+
+// LDS a workgroup may take without lowering the workgroups per CU that its
+// registers allow (four 256-thread, two 512-thread or one 1024-thread
+// workgroup; 512-hit slots hold 256-thread workgroups to two by LDS).
+template <int B, int C>
+struct LdsBudget {
+    static constexpr int value = B >= 1024 ? 163840 : (B >= 512 || C >= 512) ? 81920 : 40960;
+};
+
+// Entries of each ttf list kept in LDS (entries beyond go to HBM): what the
+// budget leaves after the rest of the workgroup's LDS, less 256 bytes.
+template <int B, int C>
+struct TtfCap {
+    static constexpr int room = LdsBudget<B, C>::value - (int)sizeof(SharedCore<B, C>) - 256;
+    static constexpr int value = room > 0 ? room / (2 * 48) : 0;
+};
+
+template <int B, int C>
+struct Shared : SharedCore<B, C> {
+    // the two ttf lists (current and next step), front part: entry i is
+    // ttf[k][0][i], ttf[k][1][i], ttf[k][2][i] (16 bytes each), so that
+    // consecutive lanes read consecutive 16-byte words
+    uint4 ttf[2][3][TtfCap<B, C>::value > 0 ? TtfCap<B, C>::value : 1];
 };
 
 template <int B, int C>
@@ -601,6 +625,68 @@ __device__ void gather_batch(Shared<B, C>& sh, const UsedFlags& used, int start0
     }
 }
 
+// A ttf list: its first TtfCap entries in LDS buffer k, the rest in HBM at
+// the same index.
+struct TtfList {
+    TtfEntry* hbm;
+    int k;
+};
+// Entries move as three 16-byte words, so that an entry stays in registers.
+// The empty asm statement ends the HBM branch with an instruction the LDS
+// branch lacks: without it the compiler sinks the two branches' accesses
+// into one flat access from a selected address, which waits on both
+// counters and parks the entry in private memory.
+static_assert(sizeof(TtfEntry) == 48, "a ttf entry is three 16-byte words");
+__device__ __forceinline__ TtfEntry ttf_unpack(uint4 a, uint4 b, uint4 c) {
+    TtfEntry e;
+    e.word = a.x;
+    e.nhits = a.y;
+    e.t0 = a.z;
+    e.t1 = a.w;
+    e.t2 = b.x;
+    e.x0 = __uint_as_float(b.y);
+    e.y0 = __uint_as_float(b.z);
+    e.z0 = __uint_as_float(b.w);
+    e.x1 = __uint_as_float(c.x);
+    e.y1 = __uint_as_float(c.y);
+    e.z1 = __uint_as_float(c.z);
+    e.pad = c.w;
+    return e;
+}
+template <int B, int C>
+__device__ __forceinline__ TtfEntry ttf_get(Shared<B, C>& sh, const TtfList& l, int i) {
+    uint4 a, b, c;
+    if (i < TtfCap<B, C>::value) {
+        a = sh.ttf[l.k][0][i];
+        b = sh.ttf[l.k][1][i];
+        c = sh.ttf[l.k][2][i];
+    } else {
+        const uint4* p = reinterpret_cast<const uint4*>(l.hbm + i);
+        a = p[0];
+        b = p[1];
+        c = p[2];
+        __asm__ volatile("");
+    }
+    return ttf_unpack(a, b, c);
+}
+template <int B, int C>
+__device__ __forceinline__ void ttf_put(Shared<B, C>& sh, const TtfList& l, int i, const TtfEntry& e) {
+    const uint4 a = make_uint4(e.word, e.nhits, e.t0, e.t1);
+    const uint4 b = make_uint4(e.t2, __float_as_uint(e.x0), __float_as_uint(e.y0), __float_as_uint(e.z0));
+    const uint4 c = make_uint4(__float_as_uint(e.x1), __float_as_uint(e.y1), __float_as_uint(e.z1), e.pad);
+    if (i < TtfCap<B, C>::value) {
+        sh.ttf[l.k][0][i] = a;
+        sh.ttf[l.k][1][i] = b;
+        sh.ttf[l.k][2][i] = c;
+    } else {
+        uint4* p = reinterpret_cast<uint4*>(l.hbm + i);
+        p[0] = a;
+        p[1] = b;
+        p[2] = c;
+        __asm__ volatile("");
+    }
+}
+
 struct StepCtx {
     int f;
     SView v0, v2, v4;  // sensors f, f-2, f-4
@@ -621,7 +707,7 @@ struct StepCtx {
 // ---------------------------------------------------------------------------
 template <int B, int C>
 __device__ void seed_batch(Shared<B, C>& sh, const StepCtx& sc, const UsedFlags& used,
-                           TtfEntry* nxt, int nv, int& n_next, int& n_tracklets, PhaseClock& pc,
+                           const TtfList& nxt, int nv, int& n_next, int& n_tracklets, PhaseClock& pc,
                            int& sel) {
     const int tid = threadIdx.x;
     const SView& v0 = sc.v0;
@@ -804,7 +890,7 @@ __device__ void seed_batch(Shared<B, C>& sh, const StepCtx& sc, const UsedFlags&
             t.y1 = v4.y[k];
             t.z1 = v4.z[k];
             t.pad = 0;
-            nxt[n_next + pos[0]] = t;
+            ttf_put(sh, nxt, n_next + pos[0], t);
         }
         n_tracklets += tot[0];
         n_next += tot[0];
@@ -866,8 +952,8 @@ __device__ __forceinline__ int n_w1_items(const Shared<B, C>& sh, int g1) {
 struct StepState {
     int n_cur, n_tracks, n_tracklets, n_weak, visits;
     int sel;        // block_scan3 buffer in turn
-    TtfEntry* cur;  // entries inserted in the previous step
-    TtfEntry* nxt;  // entries inserted in this step
+    TtfList cur;    // entries inserted in the previous step
+    TtfList nxt;    // entries inserted in this step
 };
 
 // One sensor step f: forwarding, gather, seeding, plus the windows and row
@@ -883,8 +969,8 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
     int& n_tracklets = st.n_tracklets;
     int& n_weak = st.n_weak;
     int& visits = st.visits;
-    TtfEntry*& cur = st.cur;
-    TtfEntry*& nxt = st.nxt;
+    TtfList& cur = st.cur;
+    TtfList& nxt = st.nxt;
     StepCtx sc;
     sc.f = f;
     sc.v0 = sensor_view<L>(sh, ev, f);
@@ -917,7 +1003,8 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
         bool found = false, f_new = false, f_req = false, f_weak = false;
         int bk = -1;
         if (active) {
-            t = base == 0 ? first : cur[i];  // the first tile was loaded ahead
+            if (base == 0) t = first;  // the first tile was loaded ahead
+            else t = ttf_get(sh, cur, i);
             const bool is_tl = (t.word & CLF_TTF_TRACKLET_BIT) != 0;
             const uint32_t skipped = (t.word & CLF_TTF_SKIP_MASK) >> CLF_TTF_SKIP_SHIFT;
             const float td = 1.0f / (t.z1 - t.z0);
@@ -996,7 +1083,7 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
                     n.y1 = v4.y[bk];
                     n.z1 = v4.z[bk];
                     n.pad = 0;
-                    nxt[n_next + pos[1]] = n;
+                    ttf_put(sh, nxt, n_next + pos[1], n);
                 } else {
                     atomicOr(&sh.flags, CLF_FLAG_TRACK_FULL);
                     atomicAdd(&sh.n_frozen, 1);
@@ -1005,7 +1092,7 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
                 const uint32_t skipped = (t.word & CLF_TTF_SKIP_MASK) >> CLF_TTF_SKIP_SHIFT;
                 TtfEntry n = t;
                 n.word = ((skipped + 1) << CLF_TTF_SKIP_SHIFT) | (t.word & CLF_TTF_KEEP_MASK);
-                nxt[n_next + pos[1]] = n;
+                ttf_put(sh, nxt, n_next + pos[1], n);
             } else if (f_weak) {
                 weak[n_weak + pos[2]] = make_uint4(t.t0, t.t1, t.t2, 0u);
             }
@@ -1045,7 +1132,7 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
         const int live = n_cur + n_next;
         if (live > sh.max_live) sh.max_live = live;
     }
-    TtfEntry* tmp = cur;
+    const TtfList tmp = cur;
     cur = nxt;
     nxt = tmp;
     n_cur = n_next;
@@ -1090,8 +1177,8 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
     ev.hbm_w1 = reinterpret_cast<W1Rec*>(a.cand + d.hit_off);
     ev.hbm_w2 = reinterpret_cast<W2Rec*>(ev.hbm_w1 + N);
     clf_track* tracks = a.tracks + d.track_off;
-    TtfEntry* cur = a.ttf_a + d.track_off;
-    TtfEntry* nxt = a.ttf_b + d.track_off;
+    TtfList cur = {a.ttf_a + d.track_off, 0};
+    TtfList nxt = {a.ttf_b + d.track_off, 1};
     uint4* weak = a.weak + d.track_off;
     UsedFlags used;
     used.lds = N <= kUsedBits;
@@ -1231,14 +1318,15 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
         sel = st.sel;
         cur = st.cur;
         nxt = st.nxt;
-        if (tid < n_cur) first = cur[tid];
+        if (tid < n_cur) first = ttf_get(sh, cur, tid);
     }
 
     // tail: tracklet entries of the last step become weak (Kernel.cl:771-792)
     for (int base = 0; base < n_cur; base += B) {
         const int i = base + tid;
         TtfEntry t = {};
-        const bool tl = i < n_cur && ((t = cur[i]).word & CLF_TTF_TRACKLET_BIT) != 0;
+        if (i < n_cur) t = ttf_get(sh, cur, i);
+        const bool tl = i < n_cur && (t.word & CLF_TTF_TRACKLET_BIT) != 0;
         int pos[3], tot[3];
         block_scan3<B, C>(sh, tl, false, false, pos, tot, sel);
         if (tl) weak[n_weak + pos[0]] = make_uint4(t.t0, t.t1, t.t2, 0u);

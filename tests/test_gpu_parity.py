@@ -87,10 +87,13 @@ def test_search_plain51_both_modes(gpu_ctx, oracle_mod, plain51_blobs):
 
 @pytest.fixture(scope="module")
 def config_blobs(plain51_blobs, mcdata_sorted):
-    # plain events, two mcdata events and two stress events whose largest
-    # sensors (over 256 hits) exceed a 256-hit LDS slot and are read from HBM
+    # plain events, two mcdata events, two stress events whose largest
+    # sensors (over 256 hits) exceed a 256-hit LDS slot and are read from HBM,
+    # and a dense event (1,300 tracks from one vertex, ttf ring span 2,879)
+    # whose ttf lists outgrow their LDS part in every configuration
     blobs = plain51_blobs[:6] + [mcdata_sorted["77"], mcdata_sorted["3"]]
     blobs += [generator.stress_event(i).pack() for i in (0, 7)]
+    blobs.append(generator.generate_event(77, n_tracks=1300, noise_fraction=0.02).pack())
     return blobs
 
 
