@@ -1578,7 +1578,8 @@ __device__ void serve_one(const ServeArgs& a, int w, Shared<B, C>& sh, uint64_t 
 }
 
 template <int B, int C>
-__global__ __launch_bounds__(B) void serve_kernel(ServeArgs a) {
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(MinWaves<B, C>::value)))
+void serve_kernel(ServeArgs a) {
     __shared__ Shared<B, C> sh;
     if (blockIdx.x == 0) {
         if (threadIdx.x < kWave) serve_feeder(a);
