@@ -930,6 +930,74 @@ __device__ __forceinline__ void stage_sensor(Shared<B, C>& sh, const EventView& 
     }
 }
 
+// The staging of one sensor split in two: stage_load reads the sensor's hits
+// (and each hit's left neighbour, for the bucket index) into registers a
+// step ahead, and stage_store writes them to the ring slot at the next step,
+// so that the HBM latency passes while a whole sensor step runs.
+template <int B, int C>
+struct StagedSensor {
+    static constexpr int kPer = (C + B - 1) / B;  // hits per thread
+    float x[kPer], xl[kPer], y[kPer], z[kPer];
+    float x_first, x_last;
+    int s;  // the sensor, or -1 (nothing staged: absent or read from HBM)
+};
+template <int B, int C>
+__device__ __forceinline__ void stage_load(const Shared<B, C>& sh, const EventView& ev, int s,
+                                           StagedSensor<B, C>& g) {
+    g.s = -1;
+    if (s < 0) return;
+    const int n = uni(sh.s_num[s]);
+    if (n > C) return;
+    const int b = uni(sh.s_start[s]);
+    g.s = s;
+#pragma unroll
+    for (int r = 0; r < StagedSensor<B, C>::kPer; ++r) {
+        const int i = threadIdx.x + r * B;
+        if (i < n) {
+            g.x[r] = ev.X[b + i];
+            g.xl[r] = i > 0 ? ev.X[b + i - 1] : 0.f;
+            g.y[r] = ev.Y[b + i];
+            g.z[r] = ev.Zh[b + i];
+        }
+    }
+    g.x_first = n > 0 ? ev.X[b] : 0.f;
+    g.x_last = n > 1 ? ev.X[b + n - 1] : 0.f;
+}
+template <int B, int C>
+__device__ __forceinline__ void stage_store(Shared<B, C>& sh, const StagedSensor<B, C>& g) {
+    const int s = g.s;
+    if (s < 0) return;
+    const int n = uni(sh.s_num[s]);
+    const int k = s % kRing;
+    const bool sorted = uni((int)sh.s_sorted[s]) != 0;
+    const float x0 = g.x_first;
+    const float span = n > 1 ? g.x_last - x0 : 0.f;
+    const float scale = span > 0.f ? (float)kBuckets / span : 0.f;
+    if (sorted && threadIdx.x == 0) {
+        sh.bk_x0[k] = x0;
+        sh.bk_scale[k] = scale;
+        if (n == 0)
+            for (int j = 0; j <= kBuckets; ++j) sh.bk_start[k][j] = 0;
+    }
+#pragma unroll
+    for (int r = 0; r < StagedSensor<B, C>::kPer; ++r) {
+        const int i = threadIdx.x + r * B;
+        if (i < n) {
+            sh.rx[k][i] = g.x[r];
+            sh.ry[k][i] = g.y[r];
+            sh.rz[k][i] = g.z[r];
+            if (sorted) {
+                // hit i writes start[j] for the buckets j it opens
+                const int bi = bucket_of(g.x[r], x0, scale);
+                const int bp = i > 0 ? bucket_of(g.xl[r], x0, scale) : -1;
+                for (int j = bp + 1; j <= bi; ++j) sh.bk_start[k][j] = (int16_t)i;
+                if (i == n - 1)
+                    for (int j = bi + 1; j <= kBuckets; ++j) sh.bk_start[k][j] = (int16_t)n;
+            }
+        }
+    }
+}
+
 // Clears the empty-window marks of w2 buffer k (all threads; a barrier must
 // follow before the windows that fill them).
 template <int B, int C>
@@ -1296,10 +1364,19 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
     // step so that its latency hides behind the staging of the next one
     TtfEntry first = {};
     int sel = 0;
+    // staging one step ahead holds a sensor's hits in registers across the
+    // step, which a 1024-thread workgroup (128 registers) pays in spills
+    constexpr bool kAhead = B <= 512;
+    StagedSensor<B, C> staged;
+    if (kAhead) stage_load<B, C>(sh, ev, S - 6, staged);
     for (int f = S - 1; f >= 4; --f) {
-        stage_sensor<B, C>(sh, ev, f - 5);
+        if (kAhead) stage_store<B, C>(sh, staged);  // sensor f-5
+        else stage_sensor<B, C>(sh, ev, f - 5);
         clear_empty_marks<B, C>(sh, (f - 3) & 1);  // filled by this step's windows of f-3
         __syncthreads();
+        // sensor f-6 for the next step; its slot (that of sensor f) is free
+        // from the next step on
+        if (kAhead) stage_load<B, C>(sh, ev, f - 6, staged);
         prof_mark(sh, pc, PROF_STAGE);
         prof_add(sh, pc, CNT_FWD_ENTRIES, (unsigned long long)n_cur);
         prof_add(sh, pc, CNT_FWD_TILES, (unsigned long long)((n_cur + B - 1) / B));
