@@ -155,6 +155,7 @@ struct SharedCore {
     // gather state of the current sensor step (Kernel.cl:720-721,753-754)
     uint32_t g_prev, g_shift, g_counter;
     int32_t g_nv, g_more;
+    int32_t n_listed;  // entries of the list the next step forwards
     // the current batch of gathered h0 (indices relative to sensor f)
     int32_t v_h0[kMaxVisits];
     int32_t v_first[kMaxVisits];  // h1 window start, relative to sensor f-2
@@ -576,7 +577,7 @@ struct UsedFlags {
 // Visits (relative to sensor f) are appended to v_h0 in lane order.
 // ---------------------------------------------------------------------------
 template <int B, int C>
-__device__ void gather_batch(Shared<B, C>& sh, const UsedFlags& used, int start0, uint32_t hn0) {
+__device__ int gather_batch(Shared<B, C>& sh, const UsedFlags& used, int start0, uint32_t hn0) {
     const int lane = lane_id();
     uint32_t prev = sh.g_prev, shift = sh.g_shift, counter = sh.g_counter;
     int nv = 0;
@@ -623,6 +624,7 @@ __device__ void gather_batch(Shared<B, C>& sh, const UsedFlags& used, int start0
         sh.g_nv = nv;
         sh.g_more = prev < hn0;
     }
+    return nv;
 }
 
 // A ttf list: its first TtfCap entries in LDS buffer k, the rest in HBM at
@@ -705,96 +707,103 @@ struct StepCtx {
 // skipped.  The winner minimises (fit, y, j, h2) lexicographically, which is
 // what the rows' strict-less first minima and the lowest-y tie-break pick.
 // ---------------------------------------------------------------------------
+// Window, row stops and reach of visit v: row y takes window elements 4j+y
+// and stops for good at the first whose h1 has no h2 window
+// (Kernel.cl:436-457); without h2 windows on sensor f-4 every row stops at
+// once.  The h2 windows are read eight at a time so the loads overlap.
+// Returns the part of the window a live row can reach.
+template <int B, int C>
+__device__ __forceinline__ int visit_setup(Shared<B, C>& sh, const StepCtx& sc, int v, float s1_z) {
+    const SView& v0 = sc.v0;
+    const int e0 = sh.v_h0[v];
+    const W1Rec r = sc.w1[e0];
+    const int nh = r.lo >= 0 ? r.hi - r.lo : 0;
+    int16_t d[4] = {kNoStop, kNoStop, kNoStop, kNoStop};
+    if (!sc.seeds) {
+        d[0] = d[1] = d[2] = d[3] = 0;
+    } else if (sc.w2_marks) {
+        // the empty windows of sensor f-2 are marked in a bitmask: the
+        // first marked element of each row's residue class
+        if (*sc.w2_n_empty > 0 && nh > 0) {
+            const int a0 = r.lo, a1 = r.lo + nh;
+            for (int wi = a0 >> 5; wi <= (a1 - 1) >> 5; ++wi) {
+                uint32_t m = sc.w2_marks[wi];
+                if (wi == (a0 >> 5)) m &= ~0u << (a0 & 31);
+                if (wi == ((a1 - 1) >> 5) && (a1 & 31)) m &= (1u << (a1 & 31)) - 1u;
+                while (m) {
+                    const int el = wi * 32 + __builtin_ctz(m) - a0;
+                    m &= m - 1u;
+                    if (d[el & 3] == kNoStop) d[el & 3] = (int16_t)(el >> 2);
+                }
+            }
+        }
+    } else {
+        const W2Rec* w2 = sc.w2 + r.lo;
+        int found = 0;
+        for (int b0 = 0; b0 < nh && found < 4; b0 += 8) {
+            // unconditional loads (index clamped into the window) so
+            // that all eight are in flight together
+            int16_t lo[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) lo[k] = w2[min(b0 + k, nh - 1)].lo;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int el = b0 + k;
+                if (el < nh && lo[k] < 0 && d[el & 3] == kNoStop) {
+                    d[el & 3] = (int16_t)(el >> 2);
+                    ++found;
+                }
+            }
+        }
+    }
+    int reach = 0;
+#pragma unroll
+    for (int y = 0; y < CLF_SEED_ROWS; ++y) {
+        sh.v_stop[v * CLF_SEED_ROWS + y] = d[y];
+        const int lim = d[y] == kNoStop ? nh : min(nh, CLF_SEED_ROWS * (int)d[y] + y);
+        reach = max(reach, lim);
+    }
+    sh.v_first[v] = r.lo;
+    const float z0 = v0.z[e0];
+    sh.v_x0[v] = v0.x[e0];
+    sh.v_y0[v] = v0.y[e0];
+    sh.v_z0[v] = z0;
+    sh.v_dymax[v] = CLF_PARAM_MAXYSLOPE * fabsf(s1_z - z0);
+    sh.v_best[v] = ~0ull;
+    return reach;
+}
+
+// The setup of a gathered batch, by the wave that gathered it: each visit's
+// window, row stops and reach (lane l takes visits l, l+64), and the
+// exclusive prefix of the reaches, which numbers the batch's pairs.
+template <int B, int C>
+__device__ void seed_prepare(Shared<B, C>& sh, const StepCtx& sc, int nv) {
+    const int lane = lane_id();
+    const SView& v2 = sc.v2;
+    // z of the first hit of sensor f-2 (:420-422)
+    const float s1_z = v2.n > 0 ? v2.z[0] : 0.f;
+    int running = 0;
+    for (int c0 = 0; c0 < nv; c0 += kWave) {
+        const int v = c0 + lane;
+        const int reach = v < nv ? visit_setup<B, C>(sh, sc, v, s1_z) : 0;
+        const int incl = wave_incl_scan(reach);
+        if (v < nv) sh.v_pre[v] = running + incl - reach;
+        running += __shfl(incl, kWave - 1);
+    }
+    if (lane == 0) sh.v_pre[nv] = running;
+}
+
+// Fits of the batch's (h0, h1) pairs (all threads), then the batch's
+// tracklets as ttf entries in visit order (one wave; n_next and n_tracklets
+// are advanced in that wave only).
 template <int B, int C>
 __device__ void seed_batch(Shared<B, C>& sh, const StepCtx& sc, const UsedFlags& used,
-                           const TtfList& nxt, int nv, int& n_next, int& n_tracklets, PhaseClock& pc,
-                           int& sel) {
+                           const TtfList& nxt, int nv, int& n_next, int& n_tracklets, PhaseClock& pc) {
     const int tid = threadIdx.x;
     const SView& v0 = sc.v0;
     const SView& v2 = sc.v2;
     const SView& v4 = sc.v4;
-    // z of the first hit of sensor f-2 (:420-422)
-    const float s1_z = v2.n > 0 ? v2.z[0] : 0.f;
-
-    // per h0: window, row stops, and the part of the window a live row can
-    // reach.  Row y takes window elements 4j+y and stops for good at the
-    // first whose h1 has no h2 window (Kernel.cl:436-457); without h2
-    // windows on sensor f-4 every row stops at once.  The h2 windows are
-    // read eight at a time so the loads overlap.
-    for (int v = tid; v < nv; v += B) {
-        const int e0 = sh.v_h0[v];
-        const W1Rec r = sc.w1[e0];
-        const int nh = r.lo >= 0 ? r.hi - r.lo : 0;
-        int16_t d[4] = {kNoStop, kNoStop, kNoStop, kNoStop};
-        if (!sc.seeds) {
-            d[0] = d[1] = d[2] = d[3] = 0;
-        } else if (sc.w2_marks) {
-            // the empty windows of sensor f-2 are marked in a bitmask: the
-            // first marked element of each row's residue class
-            if (*sc.w2_n_empty > 0 && nh > 0) {
-                const int a0 = r.lo, a1 = r.lo + nh;
-                for (int wi = a0 >> 5; wi <= (a1 - 1) >> 5; ++wi) {
-                    uint32_t m = sc.w2_marks[wi];
-                    if (wi == (a0 >> 5)) m &= ~0u << (a0 & 31);
-                    if (wi == ((a1 - 1) >> 5) && (a1 & 31)) m &= (1u << (a1 & 31)) - 1u;
-                    while (m) {
-                        const int el = wi * 32 + __builtin_ctz(m) - a0;
-                        m &= m - 1u;
-                        if (d[el & 3] == kNoStop) d[el & 3] = (int16_t)(el >> 2);
-                    }
-                }
-            }
-        } else {
-            const W2Rec* w2 = sc.w2 + r.lo;
-            int found = 0;
-            for (int b0 = 0; b0 < nh && found < 4; b0 += 8) {
-                // unconditional loads (index clamped into the window) so
-                // that all eight are in flight together
-                int16_t lo[8];
-#pragma unroll
-                for (int k = 0; k < 8; ++k) lo[k] = w2[min(b0 + k, nh - 1)].lo;
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const int el = b0 + k;
-                    if (el < nh && lo[k] < 0 && d[el & 3] == kNoStop) {
-                        d[el & 3] = (int16_t)(el >> 2);
-                        ++found;
-                    }
-                }
-            }
-        }
-        int reach = 0;
-#pragma unroll
-        for (int y = 0; y < CLF_SEED_ROWS; ++y) {
-            sh.v_stop[v * CLF_SEED_ROWS + y] = d[y];
-            const int lim = d[y] == kNoStop ? nh : min(nh, CLF_SEED_ROWS * (int)d[y] + y);
-            reach = max(reach, lim);
-        }
-        sh.v_first[v] = r.lo;
-        sh.v_pre[v] = reach;  // counts for now, prefix below
-        const float z0 = v0.z[e0];
-        sh.v_x0[v] = v0.x[e0];
-        sh.v_y0[v] = v0.y[e0];
-        sh.v_z0[v] = z0;
-        sh.v_dymax[v] = CLF_PARAM_MAXYSLOPE * fabsf(s1_z - z0);
-        sh.v_best[v] = ~0ull;
-    }
-    __syncthreads();
-    if (wave_id() == 0) {
-        const int lane = lane_id();
-        int running = 0;
-        for (int c0 = 0; c0 < nv; c0 += kWave) {
-            const int v = c0 + lane;
-            const int x = v < nv ? sh.v_pre[v] : 0;
-            const int incl = wave_incl_scan(x);
-            if (v < nv) sh.v_pre[v] = running + incl - x;
-            running += __shfl(incl, kWave - 1);
-        }
-        if (lane == 0) sh.v_pre[nv] = running;
-    }
-    __syncthreads();
-    const int P = sh.v_pre[nv];
-    prof_mark(sh, pc, PROF_SEED_SETUP);
+    const int P = uni(sh.v_pre[nv]);
     prof_add(sh, pc, CNT_PAIRS, (unsigned long long)P);
 
     // pairs (visit v, window element e), p = v_pre[v] + e; each thread walks
@@ -866,34 +875,38 @@ __device__ void seed_batch(Shared<B, C>& sh, const StepCtx& sc, const UsedFlags&
     prof_mark(sh, pc, PROF_SEED_EVAL);
 
     // tracklets, as ttf entries, in visit order (:550-566)
-    for (int base = 0; base < nv; base += B) {
-        const int vv = base + tid;
-        const bool acc = vv < nv && sh.v_best[vv] != ~0ull;
-        int pos[3], tot[3];
-        block_scan3<B, C>(sh, acc, false, false, pos, tot, sel);
-        if (acc) {
-            const unsigned long long key = sh.v_best[vv];
-            const int y = (int)((key >> 30) & 3ull);
-            const int j = (int)((key >> 16) & 0x3fffull);
-            const int k = (int)(key & 0xffffull);
-            const int h1 = sh.v_first[vv] + CLF_SEED_ROWS * j + y;
-            TtfEntry t;
-            t.word = CLF_TTF_TRACKLET_BIT | (uint32_t)(n_tracklets + pos[0]);
-            t.nhits = 3;
-            t.t0 = (uint32_t)(v0.start + sh.v_h0[vv]);
-            t.t1 = (uint32_t)(v2.start + h1);
-            t.t2 = (uint32_t)(v4.start + k);
-            t.x0 = v2.x[h1];
-            t.y0 = v2.y[h1];
-            t.z0 = v2.z[h1];
-            t.x1 = v4.x[k];
-            t.y1 = v4.y[k];
-            t.z1 = v4.z[k];
-            t.pad = 0;
-            ttf_put(sh, nxt, n_next + pos[0], t);
+    if (wave_id() == 0) {
+        const int lane = lane_id();
+        for (int c0 = 0; c0 < nv; c0 += kWave) {
+            const int vv = c0 + lane;
+            const bool acc = vv < nv && sh.v_best[vv] != ~0ull;
+            const uint64_t m = ballot(acc);
+            const int pos = mbcnt(m);
+            if (acc) {
+                const unsigned long long key = sh.v_best[vv];
+                const int y = (int)((key >> 30) & 3ull);
+                const int j = (int)((key >> 16) & 0x3fffull);
+                const int k = (int)(key & 0xffffull);
+                const int h1 = sh.v_first[vv] + CLF_SEED_ROWS * j + y;
+                TtfEntry t;
+                t.word = CLF_TTF_TRACKLET_BIT | (uint32_t)(n_tracklets + pos);
+                t.nhits = 3;
+                t.t0 = (uint32_t)(v0.start + sh.v_h0[vv]);
+                t.t1 = (uint32_t)(v2.start + h1);
+                t.t2 = (uint32_t)(v4.start + k);
+                t.x0 = v2.x[h1];
+                t.y0 = v2.y[h1];
+                t.z0 = v2.z[h1];
+                t.x1 = v4.x[k];
+                t.y1 = v4.y[k];
+                t.z1 = v4.z[k];
+                t.pad = 0;
+                ttf_put(sh, nxt, n_next + pos, t);
+            }
+            const int cnt = __popcll(m);
+            n_tracklets += cnt;
+            n_next += cnt;
         }
-        n_tracklets += tot[0];
-        n_next += tot[0];
     }
     prof_mark(sh, pc, PROF_SEED_EMIT);
 }
@@ -1029,8 +1042,7 @@ struct StepState {
 template <bool L, int B, int C>
 __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
                                          const UsedFlags& used, clf_track* tracks, uint4* weak,
-                                         bool degenerate, int f, StepState& st, PhaseClock& pc,
-                                         const TtfEntry& first) {
+                                         bool degenerate, int f, StepState& st, PhaseClock& pc) {
     const int tid = threadIdx.x, wave = wave_id();
     int& n_cur = st.n_cur;
     int& n_tracks = st.n_tracks;
@@ -1071,8 +1083,7 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
         bool found = false, f_new = false, f_req = false, f_weak = false;
         int bk = -1;
         if (active) {
-            if (base == 0) t = first;  // the first tile was loaded ahead
-            else t = ttf_get(sh, cur, i);
+            t = ttf_get(sh, cur, i);
             const bool is_tl = (t.word & CLF_TTF_TRACKLET_BIT) != 0;
             const uint32_t skipped = (t.word & CLF_TTF_SKIP_MASK) >> CLF_TTF_SKIP_SHIFT;
             const float td = 1.0f / (t.z1 - t.z0);
@@ -1180,25 +1191,35 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
     // (b) gather + seeding (Kernel.cl:720-764)
     const int start0 = sc.v0.start;
     const uint32_t hn0 = (uint32_t)sc.v0.n;
+    // Wave 0 gathers a batch and prepares it; after one barrier all waves
+    // fit its pairs; after another, wave 0 emits its tracklets and goes on to
+    // the next batch while the other waves wait at the next barrier.  Only
+    // wave 0 reads what the emit reads, so no barrier follows it: the other
+    // waves of the last batch leave for the next step, whose first barrier
+    // waits for wave 0.
     for (;;) {
-        if (wave == 0) gather_batch<B, C>(sh, used, start0, hn0);
+        if (wave == 0) {
+            const int nv0 = gather_batch<B, C>(sh, used, start0, hn0);
+            prof_mark(sh, pc, PROF_GATHER);
+            if (nv0 > 0) seed_prepare<B, C>(sh, sc, nv0);
+        }
         __syncthreads();
-        prof_mark(sh, pc, PROF_GATHER);
-        const int nv = sh.g_nv;
-        const bool more = sh.g_more != 0;
+        prof_mark(sh, pc, PROF_SEED_SETUP);
+        const int nv = uni(sh.g_nv);
+        const bool more = uni(sh.g_more) != 0;
         visits += nv;
         prof_add(sh, pc, CNT_VISITS, (unsigned long long)nv);
         prof_add(sh, pc, CNT_BATCHES, 1ull);
-        if (nv > 0) seed_batch<B, C>(sh, sc, used, nxt, nv, n_next, n_tracklets, pc, st.sel);
-        __syncthreads();
-        prof_mark(sh, pc, PROF_SEED_EMIT);
+        if (nv > 0) seed_batch<B, C>(sh, sc, used, nxt, nv, n_next, n_tracklets, pc);
         if (!more) break;
     }
 
-    // span of the reference's ttf ring during this step
+    // span of the reference's ttf ring during this step, and the length of
+    // the next step's list (exact in wave 0, which emitted the tracklets)
     if (tid == 0) {
         const int live = n_cur + n_next;
         if (live > sh.max_live) sh.max_live = live;
+        sh.n_listed = n_next;
     }
     const TtfList tmp = cur;
     cur = nxt;
@@ -1267,6 +1288,7 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
         sh.flags = 0;
         sh.max_live = 0;
         sh.n_frozen = 0;
+        sh.n_listed = 0;
     }
     __syncthreads();
 
@@ -1360,9 +1382,6 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
     prof_mark(sh, pc, PROF_PROLOGUE);
 
     int n_cur = 0, n_tracks = 0, n_tracklets = 0, n_weak = 0, visits = 0;
-    // each thread's entry of the next forwarding tile, loaded at the end of a
-    // step so that its latency hides behind the staging of the next one
-    TtfEntry first = {};
     int sel = 0;
     // staging one step ahead holds a sensor's hits in registers across the
     // step, which a 1024-thread workgroup (128 registers) pays in spills
@@ -1374,6 +1393,7 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
         else stage_sensor<B, C>(sh, ev, f - 5);
         clear_empty_marks<B, C>(sh, (f - 3) & 1);  // filled by this step's windows of f-3
         __syncthreads();
+        n_cur = uni(sh.n_listed);
         // sensor f-6 for the next step; its slot (that of sensor f) is free
         // from the next step on
         if (kAhead) stage_load<B, C>(sh, ev, f - 6, staged);
@@ -1384,9 +1404,9 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
         bool lds_step = true;
         for (int s2 = f; s2 >= f - 5 && s2 >= 0; --s2) lds_step = lds_step && sh.s_num[s2] <= C;
         if (lds_step)
-            run_step<true>(sh, ev, used, tracks, weak, degenerate, f, st, pc, first);
+            run_step<true>(sh, ev, used, tracks, weak, degenerate, f, st, pc);
         else
-            run_step<false>(sh, ev, used, tracks, weak, degenerate, f, st, pc, first);
+            run_step<false>(sh, ev, used, tracks, weak, degenerate, f, st, pc);
         n_cur = st.n_cur;
         n_tracks = st.n_tracks;
         n_tracklets = st.n_tracklets;
@@ -1395,8 +1415,9 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
         sel = st.sel;
         cur = st.cur;
         nxt = st.nxt;
-        if (tid < n_cur) first = ttf_get(sh, cur, tid);
     }
+    __syncthreads();
+    n_cur = uni(sh.n_listed);  // wave 0 may have emitted after the others left
 
     // tail: tracklet entries of the last step become weak (Kernel.cl:771-792)
     for (int base = 0; base < n_cur; base += B) {
