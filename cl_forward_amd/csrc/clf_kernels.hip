@@ -1185,7 +1185,10 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
         sh.g_shift = CLF_GATHER_CHUNK;
         sh.g_counter = 0;
     }
-    __syncthreads();  // used flags and windows visible
+    // used flags visible to the gather and the seeding: a tracklet made at
+    // step s that skipped sensors becomes a track at step f < s-1 and then
+    // marks its hits on sensors s-2 and s-4, which can be sensors f and f-2
+    __syncthreads();
     prof_mark(sh, pc, PROF_FORWARD);
 
     // (b) gather + seeding (Kernel.cl:720-764)

@@ -108,6 +108,25 @@ def test_launch_configs_identical(oracle_mod, config_blobs, block, lds_hits):
         _check_search(ctx, oracle_mod, config_blobs, f"block{block}/lds{lds_hits}")
 
 
+@pytest.mark.parametrize("block,lds_hits", [(256, 256), (512, 256), (512, 512), (1024, 256), (1024, 512)])
+def test_repeated_runs_identical(oracle_mod, config_blobs, block, lds_hits):
+    """A resident batch run eight times gives the oracle's bytes every time:
+    an unsynchronised read between the waves of a workgroup shows up as
+    run-to-run differences before it shows up in one run."""
+    from cl_forward_amd.search import Context
+
+    _say(f"search: block size {block}, lds slot {lds_hits} hits, 8 runs")
+    want = [oracle_mod.search_event(x)[0] for x in config_blobs]
+    with Context(0) as ctx:
+        ctx.set_config(block_size=block, lds_hits=lds_hits)
+        with ctx.batch(config_blobs * 4) as b:
+            for rep in range(8):
+                b.run(MODE_BATCHED)
+                outs, _ = b.results()
+                for i, o in enumerate(outs):
+                    assert o == want[i % len(want)], f"run {rep}, event {i}"
+
+
 @pytest.mark.parametrize("heavy,graph", [(1, 1), (3, 1), (4, 0), (64, 1)])
 def test_heavy_lane_identical(oracle_mod, config_blobs, heavy, graph):
     """The costliest events in a second launch of 512-thread workgroups
