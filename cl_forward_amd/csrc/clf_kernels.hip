@@ -61,7 +61,7 @@ enum ProfSlot {
     PROF_FORWARD = 2,     // forwarding + windows of the next step
     PROF_GATHER = 3,      // the h0 gather (wave 0)
     PROF_SEED_SETUP = 4,  // per-h0 window lookup, row stops, pair prefix
-    PROF_SEED_STOP = 5,   // (unused)
+    PROF_STAGE_WAIT = 5,  // the barrier after the staging
     PROF_SEED_EVAL = 6,   // (h0, h1) pairs: h2 range and fits
     PROF_SEED_EMIT = 7,   // ttf entries of the new tracklets
     PROF_TAIL = 8,        // tail + weak filter
@@ -83,7 +83,9 @@ struct PhaseClock {
 // Wave / block helpers
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
-__device__ __forceinline__ int wave_id() { return threadIdx.x / kWave; }
+// (the same in every lane of a wave: a scalar, so that branches on it are
+// scalar branches)
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x / kWave); }
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 // number of set bits of m below this lane
 __device__ __forceinline__ int mbcnt(uint64_t m) {
@@ -110,6 +112,11 @@ __device__ __forceinline__ int kth_set_bit(uint64_t m, int k) {
 __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
 __device__ __forceinline__ float uni(float x) {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+__device__ __forceinline__ int64_t uni(int64_t x) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)x >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 __device__ __forceinline__ int wave_incl_scan(int x) {
     const int lane = lane_id();
@@ -1237,7 +1244,15 @@ template <int B, int C>
 __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
     constexpr int W = Shared<B, C>::W;
     const int tid = threadIdx.x, lane = lane_id(), wave = wave_id();
-    const EventDesc d = a.ev[e];
+    // the descriptor, in scalar registers: the sensor loop below then runs
+    // as a uniform loop with scalar sensor indices
+    const EventDesc dv = a.ev[e];
+    EventDesc d;
+    d.blob_off = uni(dv.blob_off);
+    d.hit_off = uni(dv.hit_off);
+    d.track_off = uni(dv.track_off);
+    d.n_hits = uni(dv.n_hits);
+    d.n_sensors = uni(dv.n_sensors);
     EventResult* res = a.results + e;
     PhaseClock pc;
     pc.on = a.prof != nullptr;
@@ -1375,7 +1390,7 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
     if (S - 1 >= 4) {
         const int nw2 = n_w2_items(sh, S - 3), nw1 = n_w1_items(sh, S - 1);
         bool lds = true;
-        for (int s2 = S - 1; s2 >= S - 5 && s2 >= 0; --s2) lds = lds && sh.s_num[s2] <= C;
+        for (int s2 = S - 1; s2 >= S - 5 && s2 >= 0; --s2) lds = lds && uni(sh.s_num[s2]) <= C;
         for (int i = tid; i < nw2 + nw1; i += B) {
             if (lds) window_item<true>(sh, ev, i, S - 3, nw2, S - 1);
             else window_item<false>(sh, ev, i, S - 3, nw2, S - 1);
@@ -1391,21 +1406,25 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
     constexpr bool kAhead = B <= 512;
     StagedSensor<B, C> staged;
     if (kAhead) stage_load<B, C>(sh, ev, S - 6, staged);
-    for (int f = S - 1; f >= 4; --f) {
+    // f - 1 through readfirstlane: the compiler may copy the decrement into
+    // the arms of a lane-divergent branch in the step, and their merge would
+    // otherwise make the sensor loop, and every sensor index, per-lane
+    for (int f = S - 1; f >= 4; f = uni(f - 1)) {
         if (kAhead) stage_store<B, C>(sh, staged);  // sensor f-5
         else stage_sensor<B, C>(sh, ev, f - 5);
         clear_empty_marks<B, C>(sh, (f - 3) & 1);  // filled by this step's windows of f-3
+        prof_mark(sh, pc, PROF_STAGE);
         __syncthreads();
         n_cur = uni(sh.n_listed);
         // sensor f-6 for the next step; its slot (that of sensor f) is free
         // from the next step on
         if (kAhead) stage_load<B, C>(sh, ev, f - 6, staged);
-        prof_mark(sh, pc, PROF_STAGE);
+        prof_mark(sh, pc, PROF_STAGE_WAIT);
         prof_add(sh, pc, CNT_FWD_ENTRIES, (unsigned long long)n_cur);
         prof_add(sh, pc, CNT_FWD_TILES, (unsigned long long)((n_cur + B - 1) / B));
         StepState st = {n_cur, n_tracks, n_tracklets, n_weak, visits, sel, cur, nxt};
         bool lds_step = true;
-        for (int s2 = f; s2 >= f - 5 && s2 >= 0; --s2) lds_step = lds_step && sh.s_num[s2] <= C;
+        for (int s2 = f; s2 >= f - 5 && s2 >= 0; --s2) lds_step = lds_step && uni(sh.s_num[s2]) <= C;
         if (lds_step)
             run_step<true>(sh, ev, used, tracks, weak, degenerate, f, st, pc);
         else
@@ -1505,7 +1524,7 @@ void search_kernel(SearchArgs a) {
         __syncthreads();
         if (item >= a.n_items) break;  // every wave of the group leaves together
         const int idx = a.first_item + item;
-        const int e = a.order ? a.order[idx] : idx;
+        const int e = uni(a.order ? a.order[idx] : idx);
         process_event<B, C>(a, e, sh);
         __syncthreads();
     }

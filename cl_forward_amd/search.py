@@ -197,7 +197,7 @@ class Batch:
             stats.append({k: int(getattr(s, k)) for k, _ in EventStats._fields_ if k != "reserved"})
         return outs, stats
 
-    PROFILE_FIELDS = ("prologue", "stage", "forward", "gather", "seed_setup", "seed_stop",
+    PROFILE_FIELDS = ("prologue", "stage", "forward", "gather", "seed_setup", "stage_wait",
                       "seed_eval", "seed_emit", "tail", "total", "fwd_entries", "visits", "pairs",
                       "batches", "fwd_tiles", "h2_tests")
 
