@@ -196,3 +196,53 @@ def test_edge_cases(gpu_ctx, oracle_mod, single_blob):
         o_out, o_st = oracle_mod.search_event(blobs[i])
         assert_event_equal(outs[i], stats[i], o_out, o_st, f"edge[{i}]")
     assert outs[3] == b"" and stats[3]["flags"] & (1 << 7), "truncated blob must come back INVALID"
+
+
+def _degenerate_blobs(single_blob):
+    """Events outside the validated domain, where the kernels take the
+    reference's full formulas (no x-range cuts, every division done)."""
+    from cl_forward_amd.event import parse_event
+
+    ev = parse_event(single_blob)
+    S, N = ev.n_sensors, ev.n_hits
+    out = []
+
+    def blob(x, y, z):
+        return _blob(S, N, ev.sensor_Zs, ev.hitStarts, ev.hitNums, x, y, z, ev.hit_IDs)
+
+    def sensor(s):
+        b = int(ev.hitStarts[s])
+        return slice(b, b + int(ev.hitNums[s]))
+
+    # two sensors with the same hit z (z ranges overlap)
+    z = ev.Z.copy()
+    z[sensor(10)] = z[sensor(12)].mean()
+    z[sensor(12)] = z[sensor(12)].mean()
+    out.append(blob(ev.X, ev.Y, z))
+    # NaN and infinite coordinates
+    x, y = ev.X.copy(), ev.Y.copy()
+    x[sensor(20)][:3] = np.nan
+    y[sensor(30)][:2] = np.inf
+    out.append(blob(x, y, ev.Z))
+    # a sensor 1e-6 from its neighbour in z (ranges apart, tiny gaps)
+    z = ev.Z.copy()
+    z[sensor(15)] = np.float32(z[sensor(16)].max()) + np.float32(1e-6) * np.abs(z[sensor(16)].max())
+    out.append(blob(ev.X, ev.Y, z))
+    # coordinates at the edge of the domain (|x| near 1e5)
+    x = ev.X.copy()
+    x[sensor(25)] *= np.float32(9.0e4) / max(float(np.abs(x[sensor(25)]).max()), 1.0)
+    out.append(blob(x, ev.Y, ev.Z))
+    return out
+
+
+@pytest.mark.parametrize("block,lds_hits", [(256, 256), (512, 256), (1024, 512)])
+def test_degenerate_domain_events(oracle_mod, single_blob, block, lds_hits):
+    """Overlapping or nearly touching sensor z ranges, NaN/inf and extreme
+    coordinates: bytes, counts and flags equal to the oracle."""
+    from cl_forward_amd.search import Context
+
+    _say(f"search: degenerate-domain events, block size {block}, lds slot {lds_hits}")
+    blobs = _degenerate_blobs(single_blob)
+    with Context(0) as ctx:
+        ctx.set_config(block_size=block, lds_hits=lds_hits)
+        _check_search(ctx, oracle_mod, blobs, f"degenerate/block{block}")
