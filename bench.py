@@ -273,6 +273,13 @@ def main(argv=None):
     # device-side timings with hipEvents on the library's stream
     dev = batch.time(MODE_BATCHED, warmup=1, reps=a.steps)
     log(f"device: run {dev['ms_run']:.3f} ms, search kernel {dev['ms_search']:.3f} ms")
+    # spread of single replays (each timed on its own, after the timed region)
+    singles = sorted(batch.time(MODE_BATCHED, warmup=0, reps=1)["ms_run"] for _ in range(30))
+    spread = {"n": len(singles), "median_ms": round(singles[len(singles) // 2], 4),
+              "p10_ms": round(singles[len(singles) // 10], 4),
+              "p90_ms": round(singles[(9 * len(singles)) // 10], 4),
+              "what": "device time of single graph replays (hipEvents), after the timed region"}
+    log(f"single replays: median {spread['median_ms']} ms, p10 {spread['p10_ms']}, p90 {spread['p90_ms']}")
 
     plain = None
     if not a.no_plain:
@@ -362,6 +369,7 @@ def main(argv=None):
             "flags_seen": flags_any,
         },
         "device_ms_per_step": round(dev["ms_run"], 4),
+        "step_spread": spread,
         "plain": plain,
         "batched_vs_plain": round(value / plain["events_per_s"], 2) if plain else None,
         "end_to_end": e2e,
