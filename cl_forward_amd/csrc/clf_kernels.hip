@@ -14,8 +14,14 @@
 // LDS slots; every step stages one new sensor.  A sensor with more hits
 // than a slot holds (C, a template parameter the autotuner picks) is read
 // from HBM instead.  Candidate windows live in small LDS double buffers
-// (HBM for oversized sensors), the used flags in an LDS bitmask.  Only the ttf lists, weak entries and output
-// tracks go to HBM.
+// (HBM for oversized sensors), the used flags in an LDS bitmask, and the
+// two ttf lists in LDS up to what the configuration's occupancy leaves
+// (HBM beyond).  Weak entries and output tracks go to HBM.
+//
+// Registers.  Values that are the same in every lane are kept in scalar
+// registers (uni(): sensor tables, the event descriptor, the wave index,
+// block totals, the sensor loop counter), which holds a workgroup to 128
+// vector registers: two 512-thread or four 256-thread workgroups per CU.
 //
 // Stages:
 //   (a) candidate windows  Kernel.cl:50-178   window_h1 / window_h2: binary
