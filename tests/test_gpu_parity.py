@@ -246,3 +246,17 @@ def test_degenerate_domain_events(oracle_mod, single_blob, block, lds_hits):
     with Context(0) as ctx:
         ctx.set_config(block_size=block, lds_hits=lds_hits)
         _check_search(ctx, oracle_mod, blobs, f"degenerate/block{block}")
+
+
+@pytest.mark.parametrize("block,lds_hits", [(256, 256), (512, 256), (1024, 512)])
+def test_mcdata_all_launch_configs(oracle_mod, mcdata_sorted, mcdata_raw, block, lds_hits):
+    """All 52 real-geometry events (two hit z per sensor, sensors over 128
+    hits), presorted and as stored, in the tuned and the extreme workgroup
+    sizes: identical to the oracle."""
+    from cl_forward_amd.search import Context
+
+    _say(f"search: 104 mcdata events, block size {block}, lds slot {lds_hits}")
+    blobs = list(mcdata_sorted.values()) + list(mcdata_raw.values())
+    with Context(0) as ctx:
+        ctx.set_config(block_size=block, lds_hits=lds_hits)
+        _check_search(ctx, oracle_mod, blobs, f"mcdata/block{block}")
