@@ -76,6 +76,27 @@ def test_reference_signature_headers_compile(tmp_path):
     subprocess.run(["gcc", "-std=c99", "-fsyntax-only", f"-I{INCLUDE}", str(csrc)], check=True)
 
 
+def test_reference_caller_links_against_the_library(tmp_path):
+    """A caller of the reference's five C++ entry points links against
+    libclf.so as the integration notes describe (not run: no GPU here)."""
+    src = tmp_path / "caller.cpp"
+    src.write_text(
+        '#include <cstddef>\n#include "GpuPixelSearchByTriplet.h"\n#include "KernelInvoker.h"\n'
+        "int main() {\n"
+        "  std::vector<std::vector<uint8_t> > events, out;\n"
+        "  std::vector<const std::vector<uint8_t>*> ptrs;\n"
+        "  for (size_t i = 0; i < events.size(); ++i) ptrs.push_back(&events[i]);\n"
+        "  int r = gpuPixelSearchByTriplet(ptrs, out) + gpuPixelSearchByTripletInvocation(ptrs, out);\n"
+        "  r += invokeParallelSearch(0, 0, ptrs, out) + independent_execute(events, out);\n"
+        "  independent_post_execute(out);\n"
+        "  return r;\n}\n")
+    exe = tmp_path / "caller"
+    lib_dir = os.path.dirname(LIB)
+    subprocess.run(["g++", "-std=c++11", f"-I{INCLUDE}", str(src), "-o", str(exe), f"-L{lib_dir}", "-lclf",
+                    f"-Wl,-rpath,{lib_dir}"], check=True)
+    assert exe.exists()
+
+
 def test_status_strings_and_no_device(lib):
     assert lib.clf_status_string(0) == b"ok"
     assert lib.clf_version().startswith(b"clf")
