@@ -1138,17 +1138,24 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
                 f_weak = true;
             }
         }
+        // used flags before the scan's barrier: the gather and the seeding
+        // of this step read them after it (a tracklet that skipped sensors
+        // marks hits of sensors f and f-2 when it becomes a track)
+        if (found) {
+            used.set(v4.start + bk);
+            if (f_new) {
+                used.set((int)t.t0);
+                used.set((int)t.t1);
+                used.set((int)t.t2);
+            }
+        }
         int pos[3], tot[3];
         block_scan3<B, C>(sh, f_new, f_req, f_weak, pos, tot, st.sel);
         if (active) {
             if (found) {
                 const int b = v4.start + bk;
-                used.set(b);
                 uint32_t slot;
                 if (f_new) {
-                    used.set((int)t.t0);
-                    used.set((int)t.t1);
-                    used.set((int)t.t2);
                     slot = (uint32_t)(n_tracks + pos[0]);
                     uint32_t* r = reinterpret_cast<uint32_t*>(tracks + slot);
                     r[0] = 4;
@@ -1198,10 +1205,10 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
         sh.g_shift = CLF_GATHER_CHUNK;
         sh.g_counter = 0;
     }
-    // used flags visible to the gather and the seeding: a tracklet made at
-    // step s that skipped sensors becomes a track at step f < s-1 and then
-    // marks its hits on sensors s-2 and s-4, which can be sensors f and f-2
-    __syncthreads();
+    // No barrier here: every used flag of the forwarding was set before the
+    // last block scan's barrier, and what the forwarding writes after it
+    // (tracks, ttf entries of the next list, weak) is not read by the gather
+    // or the seeding; the barrier after wave 0's gather orders the rest.
     prof_mark(sh, pc, PROF_FORWARD);
 
     // (b) gather + seeding (Kernel.cl:720-764)
