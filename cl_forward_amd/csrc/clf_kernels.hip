@@ -924,14 +924,29 @@ __device__ void seed_batch(Shared<B, C>& sh, const StepCtx& sc, const UsedFlags&
     prof_mark(sh, pc, PROF_SEED_EMIT);
 }
 
-// Copies sensor s into its ring slot when it fits (all threads).
+// Threads that stage a sensor at the top of a step: all but wave 0 for a
+// 512-thread workgroup whose other waves cover the slot with as many hits
+// per thread, so that wave 0 emits the previous step's tracklets meanwhile
+// (the emit reads the slots of sensors f-1 and f-3, never that of f+1,
+// which the staging overwrites).  Measured: +1.5-2 % at 512 x 256; none at
+// 1024 threads, which stage straight from HBM and would spill more.
 template <int B, int C>
+struct StageThreads {
+    static constexpr int first =
+        (B == 512 && (C + B - kWave - 1) / (B - kWave) == (C + B - 1) / B) ? kWave : 0;
+    static constexpr int count = B - first;
+};
+
+// Copies sensor s into its ring slot when it fits (threads first .. B-1).
+template <int B, int C, int First = 0>
 __device__ __forceinline__ void stage_sensor(Shared<B, C>& sh, const EventView& ev, int s) {
-    if (s < 0) return;
+    constexpr int kCount = B - First;
+    const int t = (int)threadIdx.x - First;
+    if (s < 0 || t < 0) return;
     const int n = uni(sh.s_num[s]);
     if (n > C) return;
     const int b = uni(sh.s_start[s]), k = s % kRing;
-    for (int i = threadIdx.x; i < n; i += B) {
+    for (int i = t; i < n; i += kCount) {
         sh.rx[k][i] = ev.X[b + i];
         sh.ry[k][i] = ev.Y[b + i];
         sh.rz[k][i] = ev.Zh[b + i];
@@ -941,13 +956,13 @@ __device__ __forceinline__ void stage_sensor(Shared<B, C>& sh, const EventView& 
     const float x0 = n > 0 ? ev.X[b] : 0.f;
     const float span = n > 1 ? ev.X[b + n - 1] - x0 : 0.f;
     const float scale = span > 0.f ? (float)kBuckets / span : 0.f;
-    if (threadIdx.x == 0) {
+    if (t == 0) {
         sh.bk_x0[k] = x0;
         sh.bk_scale[k] = scale;
         if (n == 0)
             for (int j = 0; j <= kBuckets; ++j) sh.bk_start[k][j] = 0;
     }
-    for (int i = threadIdx.x; i < n; i += B) {
+    for (int i = t; i < n; i += kCount) {
         const int bi = bucket_of(ev.X[b + i], x0, scale);
         const int bp = i > 0 ? bucket_of(ev.X[b + i - 1], x0, scale) : -1;
         for (int j = bp + 1; j <= bi; ++j) sh.bk_start[k][j] = (int16_t)i;
@@ -962,7 +977,8 @@ __device__ __forceinline__ void stage_sensor(Shared<B, C>& sh, const EventView& 
 // so that the HBM latency passes while a whole sensor step runs.
 template <int B, int C>
 struct StagedSensor {
-    static constexpr int kPer = (C + B - 1) / B;  // hits per thread
+    static constexpr int kThreads = StageThreads<B, C>::count;
+    static constexpr int kPer = (C + kThreads - 1) / kThreads;  // hits per staging thread
     float x[kPer], xl[kPer], y[kPer], z[kPer];
     float x_first, x_last;
     int s;  // the sensor, or -1 (nothing staged: absent or read from HBM)
@@ -971,14 +987,15 @@ template <int B, int C>
 __device__ __forceinline__ void stage_load(const Shared<B, C>& sh, const EventView& ev, int s,
                                            StagedSensor<B, C>& g) {
     g.s = -1;
-    if (s < 0) return;
+    const int t = (int)threadIdx.x - StageThreads<B, C>::first;
+    if (s < 0 || t < 0) return;
     const int n = uni(sh.s_num[s]);
     if (n > C) return;
     const int b = uni(sh.s_start[s]);
     g.s = s;
 #pragma unroll
     for (int r = 0; r < StagedSensor<B, C>::kPer; ++r) {
-        const int i = threadIdx.x + r * B;
+        const int i = t + r * StagedSensor<B, C>::kThreads;
         if (i < n) {
             g.x[r] = ev.X[b + i];
             g.xl[r] = i > 0 ? ev.X[b + i - 1] : 0.f;
@@ -992,14 +1009,15 @@ __device__ __forceinline__ void stage_load(const Shared<B, C>& sh, const EventVi
 template <int B, int C>
 __device__ __forceinline__ void stage_store(Shared<B, C>& sh, const StagedSensor<B, C>& g) {
     const int s = g.s;
-    if (s < 0) return;
+    if (s < 0) return;  // also every thread outside the staging threads
+    const int t = (int)threadIdx.x - StageThreads<B, C>::first;
     const int n = uni(sh.s_num[s]);
     const int k = s % kRing;
     const bool sorted = uni((int)sh.s_sorted[s]) != 0;
     const float x0 = g.x_first;
     const float span = n > 1 ? g.x_last - x0 : 0.f;
     const float scale = span > 0.f ? (float)kBuckets / span : 0.f;
-    if (sorted && threadIdx.x == 0) {
+    if (sorted && t == 0) {
         sh.bk_x0[k] = x0;
         sh.bk_scale[k] = scale;
         if (n == 0)
@@ -1007,7 +1025,7 @@ __device__ __forceinline__ void stage_store(Shared<B, C>& sh, const StagedSensor
     }
 #pragma unroll
     for (int r = 0; r < StagedSensor<B, C>::kPer; ++r) {
-        const int i = threadIdx.x + r * B;
+        const int i = t + r * StagedSensor<B, C>::kThreads;
         if (i < n) {
             sh.rx[k][i] = g.x[r];
             sh.ry[k][i] = g.y[r];
@@ -1424,7 +1442,7 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
     // otherwise make the sensor loop, and every sensor index, per-lane
     for (int f = S - 1; f >= 4; f = uni(f - 1)) {
         if (kAhead) stage_store<B, C>(sh, staged);  // sensor f-5
-        else stage_sensor<B, C>(sh, ev, f - 5);
+        else stage_sensor<B, C, StageThreads<B, C>::first>(sh, ev, f - 5);
         clear_empty_marks<B, C>(sh, (f - 3) & 1);  // filled by this step's windows of f-3
         prof_mark(sh, pc, PROF_STAGE);
         __syncthreads();
