@@ -421,10 +421,14 @@ __device__ __forceinline__ SView sensor_view(Shared<B, C>& sh, const EventView& 
         v.x = sh.rx[k];
         v.y = sh.ry[k];
         v.z = sh.rz[k];
+        // read whether or not the sensor is sorted, so that these reads go
+        // out with the ones above instead of after them
+        const float bx0 = uni(sh.bk_x0[k]);
+        const float bscale = uni(sh.bk_scale[k]);
         if (v.sorted) {
             v.bstart = sh.bk_start[k];
-            v.bx0 = uni(sh.bk_x0[k]);
-            v.bscale = uni(sh.bk_scale[k]);
+            v.bx0 = bx0;
+            v.bscale = bscale;
         }
     } else {
         v.x = ev.X + v.start;
@@ -1454,8 +1458,11 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
         prof_add(sh, pc, CNT_FWD_ENTRIES, (unsigned long long)n_cur);
         prof_add(sh, pc, CNT_FWD_TILES, (unsigned long long)((n_cur + B - 1) / B));
         StepState st = {n_cur, n_tracks, n_tracklets, n_weak, visits, sel, cur, nxt};
+        // every sensor f .. f-5 (down to 0) in its ring slot: six independent
+        // reads (a short-circuit chain here made six LDS round trips a step)
         bool lds_step = true;
-        for (int s2 = f; s2 >= f - 5 && s2 >= 0; --s2) lds_step = lds_step && uni(sh.s_num[s2]) <= C;
+#pragma unroll
+        for (int d = 0; d < kRing; ++d) lds_step &= uni(sh.s_num[max(f - d, 0)]) <= C;
         if (lds_step)
             run_step<true>(sh, ev, used, tracks, weak, degenerate, f, st, pc);
         else
