@@ -124,14 +124,44 @@ __device__ __forceinline__ int64_t uni(int64_t x) {
     const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)x >> 32));
     return (int64_t)(((uint64_t)hi << 32) | lo);
 }
+// Inclusive prefix sum over the 64 lanes of a wave (all lanes active), in
+// DPP moves that stay in the vector ALU: shifts by 1, 2, 4, 8 within each
+// row of 16 lanes (lanes shifted in from outside the row add 0), then lane
+// 15 of rows 0 and 2 added to rows 1 and 3, then lane 31 added to rows 2
+// and 3.  A shuffle-based scan takes six LDS-pipeline round trips instead.
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
+constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
 __device__ __forceinline__ int wave_incl_scan(int x) {
-    const int lane = lane_id();
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        const int y = __shfl_up(x, d);
-        if (lane >= d) x += y;
-    }
+    x += __builtin_amdgcn_update_dpp(0, x, kDppRowShr1, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, kDppRowShr2, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, kDppRowShr4, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, kDppRowShr8, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, kDppRowBcast15, 0xa, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, kDppRowBcast31, 0xc, 0xf, false);
     return x;
+}
+// Minimum and maximum over the 64 lanes of a wave (all lanes active), the
+// same DPP pattern with +inf / -inf shifted in; lane 63 holds the result.
+template <int Ctrl, int RowMask>
+__device__ __forceinline__ float dpp_move(float ident, float x) {
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(ident), __float_as_int(x), Ctrl, RowMask, 0xf, false));
+}
+__device__ __forceinline__ void wave_min_max(float& mn, float& mx) {
+    mn = fminf(mn, dpp_move<kDppRowShr1, 0xf>(INFINITY, mn));
+    mx = fmaxf(mx, dpp_move<kDppRowShr1, 0xf>(-INFINITY, mx));
+    mn = fminf(mn, dpp_move<kDppRowShr2, 0xf>(INFINITY, mn));
+    mx = fmaxf(mx, dpp_move<kDppRowShr2, 0xf>(-INFINITY, mx));
+    mn = fminf(mn, dpp_move<kDppRowShr4, 0xf>(INFINITY, mn));
+    mx = fmaxf(mx, dpp_move<kDppRowShr4, 0xf>(-INFINITY, mx));
+    mn = fminf(mn, dpp_move<kDppRowShr8, 0xf>(INFINITY, mn));
+    mx = fmaxf(mx, dpp_move<kDppRowShr8, 0xf>(-INFINITY, mx));
+    mn = fminf(mn, dpp_move<kDppRowBcast15, 0xa>(INFINITY, mn));
+    mx = fmaxf(mx, dpp_move<kDppRowBcast15, 0xa>(-INFINITY, mx));
+    mn = fminf(mn, dpp_move<kDppRowBcast31, 0xc>(INFINITY, mn));
+    mx = fmaxf(mx, dpp_move<kDppRowBcast31, 0xc>(-INFINITY, mx));
+    mn = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mn), kWave - 1));
+    mx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mx), kWave - 1));
 }
 
 template <int B, int C>
@@ -805,7 +835,7 @@ __device__ void seed_prepare(Shared<B, C>& sh, const StepCtx& sc, int nv) {
         const int reach = v < nv ? visit_setup<B, C>(sh, sc, v, s1_z) : 0;
         const int incl = wave_incl_scan(reach);
         if (v < nv) sh.v_pre[v] = running + incl - reach;
-        running += __shfl(incl, kWave - 1);
+        running += __builtin_amdgcn_readlane(incl, kWave - 1);
     }
     if (lane == 0) sh.v_pre[nv] = running;
 }
@@ -1357,11 +1387,7 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
             mx = fmaxf(mx, z);
             if (i + 1 < n && !(x <= ev.X[b + i + 1])) unsorted = true;
         }
-#pragma unroll
-        for (int d2 = 1; d2 < kWave; d2 <<= 1) {
-            mn = fminf(mn, __shfl_xor(mn, d2));
-            mx = fmaxf(mx, __shfl_xor(mx, d2));
-        }
+        wave_min_max(mn, mx);
         const bool any_unsorted = ballot(unsorted) != 0;
         const bool any_bad = ballot(bad) != 0 || !(fabsf((float)sh.s_Z[s]) <= kLim);
         if (lane == 0) {
