@@ -372,21 +372,27 @@ __device__ __forceinline__ void x_window(float pa, float pb, float& lo, float& h
     hi = hi + (1.0e-3f + fabsf(hi) * 1.0e-6f);
 }
 
-// fitHitToTrack (Kernel.cl:15-35) with its multiply-by-bool blend.
+// fitHitToTrack (Kernel.cl:15-35) with its multiply-by-bool blend, split
+// at the hit's z: fit_at takes the predictions and the scatter denominator,
+// which depend on the candidate only through its z.
+__device__ __forceinline__ float fit_at(float x_prediction, float y_prediction, float scatterDenom,
+                                        float x2, float y2) {
+    const float dx = fabsf(x_prediction - x2);
+    const bool tolx = dx < CLF_PARAM_TOLERANCE;
+    const float dy = fabsf(y_prediction - y2);
+    const bool toly = dy < CLF_PARAM_TOLERANCE;
+    const float scatterNum = (dx * dx) + (dy * dy);
+    const float scatter = scatterNum * scatterDenom * scatterDenom;
+    const bool cond = tolx && toly && (scatter < CLF_MAX_SCATTER);
+    return (float)cond * scatter + (float)(!cond) * FLT_MAX;
+}
 __device__ __forceinline__ float fit_hit_to_track(float tx, float ty, float x0, float y0, float z0,
                                                   float h1_z, float x2, float y2, float z2) {
     const float dz = z2 - z0;
     const float x_prediction = x0 + tx * dz;
-    const float dx = fabsf(x_prediction - x2);
-    const bool tolx = dx < CLF_PARAM_TOLERANCE;
     const float y_prediction = y0 + ty * dz;
-    const float dy = fabsf(y_prediction - y2);
-    const bool toly = dy < CLF_PARAM_TOLERANCE;
-    const float scatterNum = (dx * dx) + (dy * dy);
     const float scatterDenom = 1.f / (z2 - h1_z);
-    const float scatter = scatterNum * scatterDenom * scatterDenom;
-    const bool cond = tolx && toly && (scatter < CLF_MAX_SCATTER);
-    return (float)cond * scatter + (float)(!cond) * FLT_MAX;
+    return fit_at(x_prediction, y_prediction, scatterDenom, x2, y2);
 }
 
 // One sensor's hits: coordinates in LDS (a ring slot) or in the event blob.
@@ -744,6 +750,7 @@ struct StepCtx {
     const uint32_t* w2_marks;   // their empty-window bitmask (nullptr: read w2)
     const int32_t* w2_n_empty;  // number of marks
     bool fast4;        // sensor f-4 x-sorted and the event in the supported domain
+    bool z4;           // fast4, and every hit of sensor f-4 at one nonzero z (v4.zmin)
     bool seeds;        // h1 sensor f-2 >= 4 has h2 windows (Kernel.cl:58,104)
 };
 
@@ -1127,6 +1134,9 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
     sc.w2_marks = marks ? sh.w2_empty[(f - 2) & 1] : nullptr;
     sc.w2_n_empty = &sh.w2_n_empty[(f - 2) & 1];
     sc.fast4 = sc.v4.sorted && !degenerate;
+    // zmin == zmax != 0 pins every z of the sensor to the same bits (the
+    // range is the exact minimum and maximum; only +0 and -0 compare equal)
+    sc.z4 = sc.fast4 && sc.v4.zmin == sc.v4.zmax && sc.v4.zmin != 0.f;
     sc.seeds = f - 2 >= 4;
     const SView& v4 = sc.v4;
 
@@ -1166,13 +1176,31 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
                 k = sv_lb_ge(v4, 0, v4.n, lo);
             }
             float best_fit = FLT_MAX;
-            for (; k < v4.n; ++k) {
-                const float x2 = v4.x[k], y2 = v4.y[k], z2 = v4.z[k];
-                if (x2 > hi) break;  // sorted: nothing further can pass
-                const float fit = fit_hit_to_track(tx, ty, t.x0, t.y0, t.z0, t.z1, x2, y2, z2);
-                const bool better = fit < best_fit;
-                best_fit = (float)better * fit + (float)(!better) * best_fit;
-                bk = better ? k : bk;
+            if (sc.z4) {
+                // one z on sensor f-4: the predictions and the scatter
+                // denominator are the same for every candidate
+                const float zc = v4.zmin;
+                const float dz = zc - t.z0;
+                const float xp = t.x0 + tx * dz;
+                const float yp = t.y0 + ty * dz;
+                const float sd = 1.f / (zc - t.z1);
+                for (; k < v4.n; ++k) {
+                    const float x2 = v4.x[k], y2 = v4.y[k];
+                    if (x2 > hi) break;  // sorted: nothing further can pass
+                    const float fit = fit_at(xp, yp, sd, x2, y2);
+                    const bool better = fit < best_fit;
+                    best_fit = (float)better * fit + (float)(!better) * best_fit;
+                    bk = better ? k : bk;
+                }
+            } else {
+                for (; k < v4.n; ++k) {
+                    const float x2 = v4.x[k], y2 = v4.y[k], z2 = v4.z[k];
+                    if (x2 > hi) break;  // sorted: nothing further can pass
+                    const float fit = fit_hit_to_track(tx, ty, t.x0, t.y0, t.z0, t.z1, x2, y2, z2);
+                    const bool better = fit < best_fit;
+                    best_fit = (float)better * fit + (float)(!better) * best_fit;
+                    bk = better ? k : bk;
+                }
             }
             if (best_fit != FLT_MAX) {
                 if (bk >= 0) {
