@@ -898,22 +898,47 @@ __device__ void seed_batch(Shared<B, C>& sh, const StepCtx& sc, const UsedFlags&
         }
         float best = FLT_MAX;
         int bk = -1;
-        for (; k < k1; ++k) {
-            const float x2 = v4.x[k], y2 = v4.y[k], z2 = v4.z[k];
-            if (x2 > hi) break;  // sorted: nothing further can pass
-            if (pc.on) ++n_h2;
-            const float z2_tz = (z2 - z0) * dz_inverted;
+        if (sc.z4) {
+            // one z on sensor f-4: the predictions are the same for every
+            // candidate, and so is the scatter denominator
+            const float zc = v4.zmin;
+            const float z2_tz = (zc - z0) * dz_inverted;
             const float x = x0 + (x1 - x0) * z2_tz;
             const float yy = y0 + (y1 - y0) * z2_tz;
-            const float dx = x - x2;
-            const float dy = yy - y2;
-            if (fabsf(dx) < CLF_PARAM_TOLERANCE && fabsf(dy) < CLF_PARAM_TOLERANCE) {
-                const float scatterNum = (dx * dx) + (dy * dy);
-                const float scatterDenom = 1.f / (z2 - z1);
-                const float scatter = scatterNum * scatterDenom * scatterDenom;
-                if (scatter < CLF_MAX_SCATTER && scatter < best) {
-                    best = scatter;
-                    bk = k;
+            for (; k < k1; ++k) {
+                const float x2 = v4.x[k], y2 = v4.y[k];
+                if (x2 > hi) break;  // sorted: nothing further can pass
+                if (pc.on) ++n_h2;
+                const float dx = x - x2;
+                const float dy = yy - y2;
+                if (fabsf(dx) < CLF_PARAM_TOLERANCE && fabsf(dy) < CLF_PARAM_TOLERANCE) {
+                    const float scatterNum = (dx * dx) + (dy * dy);
+                    const float scatterDenom = 1.f / (zc - z1);
+                    const float scatter = scatterNum * scatterDenom * scatterDenom;
+                    if (scatter < CLF_MAX_SCATTER && scatter < best) {
+                        best = scatter;
+                        bk = k;
+                    }
+                }
+            }
+        } else {
+            for (; k < k1; ++k) {
+                const float x2 = v4.x[k], y2 = v4.y[k], z2 = v4.z[k];
+                if (x2 > hi) break;  // sorted: nothing further can pass
+                if (pc.on) ++n_h2;
+                const float z2_tz = (z2 - z0) * dz_inverted;
+                const float x = x0 + (x1 - x0) * z2_tz;
+                const float yy = y0 + (y1 - y0) * z2_tz;
+                const float dx = x - x2;
+                const float dy = yy - y2;
+                if (fabsf(dx) < CLF_PARAM_TOLERANCE && fabsf(dy) < CLF_PARAM_TOLERANCE) {
+                    const float scatterNum = (dx * dx) + (dy * dy);
+                    const float scatterDenom = 1.f / (z2 - z1);
+                    const float scatter = scatterNum * scatterDenom * scatterDenom;
+                    if (scatter < CLF_MAX_SCATTER && scatter < best) {
+                        best = scatter;
+                        bk = k;
+                    }
                 }
             }
         }
