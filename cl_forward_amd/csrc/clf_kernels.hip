@@ -1148,16 +1148,12 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
     int& visits = st.visits;
     TtfList& cur = st.cur;
     TtfList& nxt = st.nxt;
+    // what the forwarding needs (sensor f-4) first; the views of f and f-2
+    // and the window tables follow the forwarding, so that the window and
+    // forwarding chains do not wait for their reads
     StepCtx sc;
     sc.f = f;
-    sc.v0 = sensor_view<L>(sh, ev, f);
-    sc.v2 = sensor_view<L>(sh, ev, f - 2);
     sc.v4 = sensor_view<L>(sh, ev, f - 4);
-    sc.w1 = w1_of<L>(sh, ev, f);
-    sc.w2 = w2_of<L>(sh, ev, f - 2);
-    const bool marks = f - 2 >= 0 && (L || sh.s_num[f - 2] <= C);
-    sc.w2_marks = marks ? sh.w2_empty[(f - 2) & 1] : nullptr;
-    sc.w2_n_empty = &sh.w2_n_empty[(f - 2) & 1];
     sc.fast4 = sc.v4.sorted && !degenerate;
     // zmin == zmax != 0 pins every z of the sensor to the same bits (the
     // range is the exact minimum and maximum; only +0 and -0 compare equal)
@@ -1310,6 +1306,13 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
         sh.g_shift = CLF_GATHER_CHUNK;
         sh.g_counter = 0;
     }
+    sc.v0 = sensor_view<L>(sh, ev, f);
+    sc.v2 = sensor_view<L>(sh, ev, f - 2);
+    sc.w1 = w1_of<L>(sh, ev, f);
+    sc.w2 = w2_of<L>(sh, ev, f - 2);
+    const bool marks = f - 2 >= 0 && (L || sh.s_num[f - 2] <= C);
+    sc.w2_marks = marks ? sh.w2_empty[(f - 2) & 1] : nullptr;
+    sc.w2_n_empty = &sh.w2_n_empty[(f - 2) & 1];
     // No barrier here: every used flag of the forwarding was set before the
     // last block scan's barrier, and what the forwarding writes after it
     // (tracks, ttf entries of the next list, weak) is not read by the gather
