@@ -52,7 +52,7 @@ namespace clf {
 
 constexpr int kWave = 64;
 constexpr int kMaxVisits = 128;   // gathered h0 handled per seeding batch (multiple of 64)
-constexpr int kRing = 6;          // LDS sensor slots: f .. f-5
+constexpr int kStepSensors = 6;   // sensors a step reads: f .. f-5
 constexpr int kUsedBits = 32768;  // events up to this many hits keep used flags in LDS
 constexpr float kLim = 1.0e5f;    // |coordinate| bound of the supported domain
 constexpr int16_t kNoStop = 0x7fff;
@@ -164,6 +164,25 @@ __device__ __forceinline__ void wave_min_max(float& mn, float& mx) {
     mx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mx), kWave - 1));
 }
 
+// Threads that stage sensors: all but wave 0 for a 512-thread workgroup
+// whose other waves cover a slot with as many hits per thread.  Those
+// waves store the next step's sensor f-6 while wave 0 gathers and sets up
+// the seeding batch alone, into a seventh ring slot (the step reads six
+// sensors, f .. f-5, and the slot of f+1 = f-6 mod 7 is free then).  Other
+// configurations stage sensor f-5 with all threads at the top of step f,
+// into the slot of f+1 (six slots).
+template <int B, int C>
+struct StageThreads {
+    static constexpr int first =
+        (B == 512 && (C + B - kWave - 1) / (B - kWave) == (C + B - 1) / B) ? kWave : 0;
+    static constexpr int count = B - first;
+    static constexpr bool during_gather = first > 0;
+};
+template <int B, int C>
+struct Ring {
+    static constexpr int slots = StageThreads<B, C>::during_gather ? kStepSensors + 1 : kStepSensors;
+};
+
 template <int B, int C>
 struct SharedCore {
     static constexpr int W = B / kWave;
@@ -175,13 +194,13 @@ struct SharedCore {
     float s_zmax[CLF_MAX_SENSORS];
     int32_t s_sorted[CLF_MAX_SENSORS];
     // ring of sensor slots: hit coordinates of sensors f .. f-5
-    float rx[kRing][C];
-    float ry[kRing][C];
-    float rz[kRing][C];
+    float rx[Ring<B, C>::slots][C];
+    float ry[Ring<B, C>::slots][C];
+    float rz[Ring<B, C>::slots][C];
     // x-bucket index of each slot (sorted sensors): the first hit of bucket
     // >= j, bucket(x) = clamp(floor((x - x0) * scale))
-    float bk_x0[kRing], bk_scale[kRing];
-    int16_t bk_start[kRing][kBuckets + 1];
+    float bk_x0[Ring<B, C>::slots], bk_scale[Ring<B, C>::slots];
+    int16_t bk_start[Ring<B, C>::slots][kBuckets + 1];
     // windows: sensor g's h1 windows in w1[g & 1], its h2 windows in
     // w2[g & 1] (each buffer lives from its write to its last read)
     W1Rec w1[2][C];
@@ -453,7 +472,7 @@ __device__ __forceinline__ SView sensor_view(Shared<B, C>& sh, const EventView& 
     v.bx0 = 0.f;
     v.bscale = 0.f;
     if (L || v.n <= C) {
-        const int k = s % kRing;
+        const int k = s % Ring<B, C>::slots;
         v.x = sh.rx[k];
         v.y = sh.ry[k];
         v.z = sh.rz[k];
@@ -990,19 +1009,6 @@ __device__ void seed_batch(Shared<B, C>& sh, const StepCtx& sc, const UsedFlags&
     prof_mark(sh, pc, PROF_SEED_EMIT);
 }
 
-// Threads that stage a sensor at the top of a step: all but wave 0 for a
-// 512-thread workgroup whose other waves cover the slot with as many hits
-// per thread, so that wave 0 emits the previous step's tracklets meanwhile
-// (the emit reads the slots of sensors f-1 and f-3, never that of f+1,
-// which the staging overwrites).  Measured: +1.5-2 % at 512 x 256; none at
-// 1024 threads, which stage straight from HBM and would spill more.
-template <int B, int C>
-struct StageThreads {
-    static constexpr int first =
-        (B == 512 && (C + B - kWave - 1) / (B - kWave) == (C + B - 1) / B) ? kWave : 0;
-    static constexpr int count = B - first;
-};
-
 // Copies sensor s into its ring slot when it fits (threads first .. B-1).
 template <int B, int C, int First = 0>
 __device__ __forceinline__ void stage_sensor(Shared<B, C>& sh, const EventView& ev, int s) {
@@ -1011,7 +1017,7 @@ __device__ __forceinline__ void stage_sensor(Shared<B, C>& sh, const EventView& 
     if (s < 0 || t < 0) return;
     const int n = uni(sh.s_num[s]);
     if (n > C) return;
-    const int b = uni(sh.s_start[s]), k = s % kRing;
+    const int b = uni(sh.s_start[s]), k = s % Ring<B, C>::slots;
     for (int i = t; i < n; i += kCount) {
         sh.rx[k][i] = ev.X[b + i];
         sh.ry[k][i] = ev.Y[b + i];
@@ -1078,7 +1084,7 @@ __device__ __forceinline__ void stage_store(Shared<B, C>& sh, const StagedSensor
     if (s < 0) return;  // also every thread outside the staging threads
     const int t = (int)threadIdx.x - StageThreads<B, C>::first;
     const int n = uni(sh.s_num[s]);
-    const int k = s % kRing;
+    const int k = s % Ring<B, C>::slots;
     const bool sorted = uni((int)sh.s_sorted[s]) != 0;
     const float x0 = g.x_first;
     const float span = n > 1 ? g.x_last - x0 : 0.f;
@@ -1139,7 +1145,8 @@ struct StepState {
 template <bool L, int B, int C>
 __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
                                          const UsedFlags& used, clf_track* tracks, uint4* weak,
-                                         bool degenerate, int f, StepState& st, PhaseClock& pc) {
+                                         bool degenerate, int f, StepState& st,
+                                         const StagedSensor<B, C>& staged, PhaseClock& pc) {
     const int tid = threadIdx.x, wave = wave_id();
     int& n_cur = st.n_cur;
     int& n_tracks = st.n_tracks;
@@ -1328,12 +1335,16 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
     // wave 0 reads what the emit reads, so no barrier follows it: the other
     // waves of the last batch leave for the next step, whose first barrier
     // waits for wave 0.
+    bool first_batch = true;
     for (;;) {
         if (wave == 0) {
             const int nv0 = gather_batch<B, C>(sh, used, start0, hn0);
             prof_mark(sh, pc, PROF_GATHER);
             if (nv0 > 0) seed_prepare<B, C>(sh, sc, nv0);
+        } else if (StageThreads<B, C>::during_gather && first_batch) {
+            stage_store<B, C>(sh, staged);  // sensor f-6, for the next step
         }
+        first_batch = false;
         __syncthreads();
         prof_mark(sh, pc, PROF_SEED_SETUP);
         const int nv = uni(sh.g_nv);
@@ -1468,8 +1479,10 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
         if (sh.s_num[f] > 0 && zs0 >= sh.s_zmin[f] && zs0 <= sh.s_zmax[f])
             atomicOr(&sh.flags, CLF_FLAG_DEGENERATE_Z);
     }
-    // the ring starts with sensors S-1 .. S-5
-    for (int s = S - 1; s >= S - 5 && s >= 0; --s) stage_sensor<B, C>(sh, ev, s);
+    // the ring starts with sensors S-1 .. S-5 (S-6 too when the staging runs
+    // during the gather, one step earlier)
+    constexpr int kEarly = StageThreads<B, C>::during_gather ? 1 : 0;
+    for (int s = S - 1; s >= S - 5 - kEarly && s >= 0; --s) stage_sensor<B, C>(sh, ev, s);
     clear_empty_marks<B, C>(sh, 0);
     clear_empty_marks<B, C>(sh, 1);
     __syncthreads();
@@ -1522,19 +1535,21 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
     // step, which a 1024-thread workgroup (128 registers) pays in spills
     constexpr bool kAhead = B <= 512;
     StagedSensor<B, C> staged;
-    if (kAhead) stage_load<B, C>(sh, ev, S - 6, staged);
+    staged.s = -1;
+    if (kAhead && !kEarly) stage_load<B, C>(sh, ev, S - 6, staged);
     // f - 1 through readfirstlane: the compiler may copy the decrement into
     // the arms of a lane-divergent branch in the step, and their merge would
     // otherwise make the sensor loop, and every sensor index, per-lane
     for (int f = S - 1; f >= 4; f = uni(f - 1)) {
-        if (kAhead) stage_store<B, C>(sh, staged);  // sensor f-5
-        else stage_sensor<B, C, StageThreads<B, C>::first>(sh, ev, f - 5);
+        if (kAhead && !kEarly) stage_store<B, C>(sh, staged);  // sensor f-5
+        else if (!kAhead) stage_sensor<B, C, StageThreads<B, C>::first>(sh, ev, f - 5);
         clear_empty_marks<B, C>(sh, (f - 3) & 1);  // filled by this step's windows of f-3
         prof_mark(sh, pc, PROF_STAGE);
         __syncthreads();
         n_cur = uni(sh.n_listed);
-        // sensor f-6 for the next step; its slot (that of sensor f) is free
-        // from the next step on
+        // sensor f-6 for the next step, stored at the top of the next step
+        // into the slot of sensor f, or during this step's gather into that
+        // of f+1 (seven slots)
         if (kAhead) stage_load<B, C>(sh, ev, f - 6, staged);
         prof_mark(sh, pc, PROF_STAGE_WAIT);
         prof_add(sh, pc, CNT_FWD_ENTRIES, (unsigned long long)n_cur);
@@ -1544,11 +1559,11 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
         // reads (a short-circuit chain here made six LDS round trips a step)
         bool lds_step = true;
 #pragma unroll
-        for (int d = 0; d < kRing; ++d) lds_step &= uni(sh.s_num[max(f - d, 0)]) <= C;
+        for (int d = 0; d < kStepSensors; ++d) lds_step &= uni(sh.s_num[max(f - d, 0)]) <= C;
         if (lds_step)
-            run_step<true>(sh, ev, used, tracks, weak, degenerate, f, st, pc);
+            run_step<true>(sh, ev, used, tracks, weak, degenerate, f, st, staged, pc);
         else
-            run_step<false>(sh, ev, used, tracks, weak, degenerate, f, st, pc);
+            run_step<false>(sh, ev, used, tracks, weak, degenerate, f, st, staged, pc);
         n_cur = st.n_cur;
         n_tracks = st.n_tracks;
         n_tracklets = st.n_tracklets;
