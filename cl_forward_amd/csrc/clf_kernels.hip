@@ -165,16 +165,16 @@ __device__ __forceinline__ void wave_min_max(float& mn, float& mx) {
 }
 
 // Threads that stage sensors: all but wave 0 for a 512-thread workgroup
-// whose other waves cover a slot with as many hits per thread.  Those
-// waves store the next step's sensor f-6 while wave 0 gathers and sets up
-// the seeding batch alone, into a seventh ring slot (the step reads six
-// sensors, f .. f-5, and the slot of f+1 = f-6 mod 7 is free then).  Other
-// configurations stage sensor f-5 with all threads at the top of step f,
-// into the slot of f+1 (six slots).
+// whose other waves cover a slot with as many hits per thread, and for a
+// 1024-thread one.  Those waves store the next step's sensor f-6 while
+// wave 0 gathers and sets up the seeding batch alone, into a seventh ring
+// slot (the step reads six sensors, f .. f-5, and the slot of f+1 = f-6
+// mod 7 is free then).  Other configurations stage sensor f-5 with all
+// threads at the top of step f, into the slot of f+1 (six slots).
 template <int B, int C>
 struct StageThreads {
     static constexpr int first =
-        (B == 512 && (C + B - kWave - 1) / (B - kWave) == (C + B - 1) / B) ? kWave : 0;
+        ((B == 512 && (C + B - kWave - 1) / (B - kWave) == (C + B - 1) / B) || B == 1024) ? kWave : 0;
     static constexpr int count = B - first;
     static constexpr bool during_gather = first > 0;
 };
@@ -1342,7 +1342,10 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
             prof_mark(sh, pc, PROF_GATHER);
             if (nv0 > 0) seed_prepare<B, C>(sh, sc, nv0);
         } else if (StageThreads<B, C>::during_gather && first_batch) {
-            stage_store<B, C>(sh, staged);  // sensor f-6, for the next step
+            // sensor f-6, for the next step: from registers loaded a step
+            // ahead, or (1024 threads) straight from HBM
+            if (B <= 512) stage_store<B, C>(sh, staged);
+            else stage_sensor<B, C, StageThreads<B, C>::first>(sh, ev, f - 6);
         }
         first_batch = false;
         __syncthreads();
@@ -1541,8 +1544,10 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
     // the arms of a lane-divergent branch in the step, and their merge would
     // otherwise make the sensor loop, and every sensor index, per-lane
     for (int f = S - 1; f >= 4; f = uni(f - 1)) {
-        if (kAhead && !kEarly) stage_store<B, C>(sh, staged);  // sensor f-5
-        else if (!kAhead) stage_sensor<B, C, StageThreads<B, C>::first>(sh, ev, f - 5);
+        if (!kEarly) {  // sensor f-5
+            if (kAhead) stage_store<B, C>(sh, staged);
+            else stage_sensor<B, C>(sh, ev, f - 5);
+        }
         clear_empty_marks<B, C>(sh, (f - 3) & 1);  // filled by this step's windows of f-3
         prof_mark(sh, pc, PROF_STAGE);
         __syncthreads();
