@@ -1180,8 +1180,12 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
     // (c) forwarding of the ttf entries of step f+1 (Kernel.cl:210-367)
     int n_next = 0;
     for (int base = 0; base < n_cur; base += B) {
-        const int i = base + tid;
-        const bool active = i < n_cur;
+        // a tile that fits on waves 1.. leaves wave 0 out (entry i on thread
+        // i + 64, the scan order unchanged), so that wave 0 goes from the
+        // scan straight to the gather, with no entries of its own to write
+        const int shift = (B > kWave && n_cur - base <= B - kWave) ? kWave : 0;
+        const int i = base + tid - shift;
+        const bool active = i >= base && i < n_cur;
         TtfEntry t = {};
         bool found = false, f_new = false, f_req = false, f_weak = false;
         int bk = -1;
