@@ -897,13 +897,17 @@ __device__ void seed_batch(Shared<B, C>& sh, const StepCtx& sc, const UsedFlags&
         const int e = p - sh.v_pre[v];
         const int y = e & (CLF_SEED_ROWS - 1);
         const int j = e / CLF_SEED_ROWS;
-        if (j >= sh.v_stop[v * CLF_SEED_ROWS + y]) continue;
+        // the visit's reads, then the h1 reads, each set in flight together
+        // before the tests that may skip the pair (e < the visit's reach <=
+        // its window size, so h1 is a hit of sensor f-2 in any case)
+        const int stop = sh.v_stop[v * CLF_SEED_ROWS + y];
         const int h1 = sh.v_first[v] + e;
-        if (used.get(v2.start + h1)) continue;
-        const float x0 = sh.v_x0[v], y0 = sh.v_y0[v], z0 = sh.v_z0[v];
+        const float x0 = sh.v_x0[v], y0 = sh.v_y0[v], z0 = sh.v_z0[v], dymax = sh.v_dymax[v];
+        const bool h1_used = used.get(v2.start + h1);
         const float x1 = v2.x[h1], y1 = v2.y[h1], z1 = v2.z[h1];
-        if (!(fabsf(y1 - y0) < sh.v_dymax[v])) continue;  // :509, independent of h2
         const W2Rec c = sc.w2[h1];
+        if (j >= stop || h1_used) continue;
+        if (!(fabsf(y1 - y0) < dymax)) continue;  // :509, independent of h2
         const float dz_inverted = 1.f / (z1 - z0);
         int k = c.lo;
         const int k1 = c.hi;
