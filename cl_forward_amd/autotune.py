@@ -34,6 +34,9 @@ HEAVY_BLOCKS = (512, 1024)
 # a second launch is kept only when it beats the single launch by this
 # fraction (run-to-run spread of the timing is a few per cent)
 HEAVY_MARGIN = 0.05
+# a grid beyond the resident workgroups only if it wins by this much: each
+# extra workgroup pays the kernel's start-up (its spill stores) in HBM traffic
+GRID_MARGIN = 0.01
 
 
 def load_tuned(num_cus: int, workload: str | None = None) -> dict | None:
@@ -51,12 +54,21 @@ def load_tuned(num_cus: int, workload: str | None = None) -> dict | None:
 
 
 def pick_best(table: list[dict]) -> dict:
-    """Fastest row; a heavy-event lane only if it wins by HEAVY_MARGIN."""
-    single = min((r for r in table if r["heavy_events"] == 0), key=lambda r: r["ms_run"])
-    best = min(table, key=lambda r: r["ms_run"])
-    if best["heavy_events"] and best["ms_run"] > (1.0 - HEAVY_MARGIN) * single["ms_run"]:
-        return single
-    return best
+    """Fastest row; among single launches of the fastest shape, the smallest
+    grid within GRID_MARGIN (0 = the resident workgroups); a heavy-event lane
+    only if it wins by HEAVY_MARGIN."""
+    singles = [r for r in table if r["heavy_events"] == 0]
+    fastest = min(singles, key=lambda r: r["ms_run"])
+    near = [r for r in singles
+            if r["block_size"] == fastest["block_size"] and r["lds_hits"] == fastest["lds_hits"]
+            and r["ms_run"] <= (1.0 + GRID_MARGIN) * fastest["ms_run"]]
+    single = min(near, key=lambda r: (r["grid_size"], r["ms_run"]))
+    heavy = [r for r in table if r["heavy_events"]]
+    if heavy:
+        best_heavy = min(heavy, key=lambda r: r["ms_run"])
+        if best_heavy["ms_run"] <= (1.0 - HEAVY_MARGIN) * fastest["ms_run"]:
+            return best_heavy
+    return single
 
 
 def tune(ctx, batch, reps: int = 5, warmup: int = 1, log=print) -> dict:

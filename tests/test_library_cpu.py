@@ -312,3 +312,20 @@ def test_autotune_keeps_single_launch_within_margin():
     assert pick_best([single, near]) is single
     assert pick_best([single, near, far]) is far
     assert pick_best([row(2.0), single, row(1.5, block=512)]) is single
+
+
+def test_autotune_prefers_resident_grid_within_margin():
+    from cl_forward_amd.autotune import GRID_MARGIN, pick_best
+
+    def row(ms, grid, block=1024):
+        return {"block_size": block, "grid_size": grid, "lds_hits": 512, "heavy_events": 0,
+                "heavy_block": 512, "ms_run": ms}
+
+    big = row(1.000, 1024)
+    resident = row(1.000 * (1.0 + GRID_MARGIN / 2), 0)
+    slow_resident = row(1.000 * (1.0 + 2 * GRID_MARGIN), 0)
+    other_shape = row(1.001, 0, block=512)
+    assert pick_best([big, resident]) is resident
+    assert pick_best([big, slow_resident]) is big
+    # the tie-break stays within the fastest launch shape
+    assert pick_best([big, other_shape]) is big
