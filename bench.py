@@ -171,7 +171,7 @@ def main(argv=None):
     ap.add_argument("--no-serve", action="store_true", help="skip the serving runs")
     ap.add_argument("--serve-only", action="store_true", help="only the serving runs")
     ap.add_argument("--serve-events", type=int, default=100000)
-    ap.add_argument("--serve-rates", type=float, nargs="+", default=[0.25, 0.5, 0.75, 1.0])
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[0.25, 0.4, 0.5, 0.75, 1.0])
     ap.add_argument("--serve-engine", choices=["resident", "batches"], default="resident")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-batch", type=int, default=256)
@@ -380,6 +380,11 @@ def main(argv=None):
         "vs_cpu_baseline": round(value / cpu["value"], 1) if cpu else None,
         "serving_p50_ms_at_50pct": at50["p50_ms"] if at50 else None,
         "serving_p99_ms_at_50pct": at50["p99_ms"] if at50 else None,
+        # the highest measured fraction of the batched peak served with p99
+        # under 2 ms and nothing refused
+        "serving_max_fraction_p99_under_2ms": max(
+            (r["fraction_of_peak"] for r in serving["rates"]
+             if r["p99_ms"] < 2.0 and r["rejected"] == 0), default=None) if serving else None,
         "serving": serving,
     }
     batch.close()
