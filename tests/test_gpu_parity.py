@@ -198,6 +198,42 @@ def test_edge_cases(gpu_ctx, oracle_mod, single_blob):
     assert outs[3] == b"" and stats[3]["flags"] & (1 << 7), "truncated blob must come back INVALID"
 
 
+def _few_sensor_blobs(single_blob):
+    """Events of 5 to 9 sensors (the first k of the synthetic event, and 7
+    from its middle): around the ring of six or seven sensor slots, where
+    the first steps stage nothing or the last sensor."""
+    from cl_forward_amd.event import parse_event
+
+    ev = parse_event(single_blob)
+    out = []
+    for k in range(5, 10):
+        n = int(ev.hitNums[:k].sum())
+        out.append(_blob(k, n, ev.sensor_Zs[:k], ev.hitStarts[:k], ev.hitNums[:k], ev.X[:n], ev.Y[:n],
+                         ev.Z[:n], ev.hit_IDs[:n]))
+    s0, k = 20, 7
+    b0 = int(ev.hitStarts[s0])
+    nums = ev.hitNums[s0:s0 + k]
+    n = int(nums.sum())
+    starts = np.asarray(ev.hitStarts[s0:s0 + k], np.int64) - b0
+    out.append(_blob(k, n, ev.sensor_Zs[s0:s0 + k], starts, nums, ev.X[b0:b0 + n], ev.Y[b0:b0 + n],
+                     ev.Z[b0:b0 + n], ev.hit_IDs[b0:b0 + n]))
+    return out
+
+
+@pytest.mark.parametrize("block,lds_hits", [(64, 256), (256, 256), (512, 256), (512, 512), (1024, 512)])
+def test_few_sensor_events(oracle_mod, single_blob, block, lds_hits):
+    """5 to 9 sensors, in launch configurations that stage at the top of a
+    step into six slots and during the gather into seven: identical to the
+    oracle."""
+    from cl_forward_amd.search import Context
+
+    _say(f"search: 5-9 sensor events, block size {block}, lds slot {lds_hits}")
+    blobs = _few_sensor_blobs(single_blob)
+    with Context(0) as ctx:
+        ctx.set_config(block_size=block, lds_hits=lds_hits)
+        _check_search(ctx, oracle_mod, blobs, f"few-sensors/block{block}")
+
+
 def _degenerate_blobs(single_blob):
     """Events outside the validated domain, where the kernels take the
     reference's full formulas (no x-range cuts, every division done)."""
