@@ -33,7 +33,7 @@ HEAVY_EVENTS = (0, 8, 16, 32, 64, 128)
 HEAVY_BLOCKS = (512, 1024)
 # a second launch is kept only when it beats the single launch by this
 # fraction (run-to-run spread of the timing is a few per cent)
-HEAVY_MARGIN = 0.02
+HEAVY_MARGIN = 0.05
 # a grid beyond the resident workgroups only if it wins by this much: each
 # extra workgroup pays the kernel's start-up (its spill stores) in HBM traffic
 GRID_MARGIN = 0.01
