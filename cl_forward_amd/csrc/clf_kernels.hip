@@ -11,7 +11,9 @@
 // Data placement.  Step f reads hits of sensors f (h0), f-2 (h1), f-4 (h2
 // and the forwarding target) and prepares sensors f-1, f-3 and f-5 for the
 // next step, so the hit coordinates of sensors f .. f-5 sit in a ring of six
-// LDS slots; every step stages one new sensor.  A sensor with more hits
+// LDS slots; every step stages one new sensor (512 x 256 and 1024-thread
+// workgroups keep a seventh slot, filled with the next step's sensor while
+// wave 0 gathers alone; see StageThreads).  A sensor with more hits
 // than a slot holds (C, a template parameter the autotuner picks) is read
 // from HBM instead.  Candidate windows live in small LDS double buffers
 // (HBM for oversized sensors), the used flags in an LDS bitmask, and the
