@@ -180,6 +180,10 @@ def load():
         "clf_server_destroy": (None, [vp]),
         "clf_server_submit": (ctypes.c_int, [vp, u8p, ctypes.c_size_t, ctypes.c_uint64,
                                              ctypes.POINTER(ctypes.c_uint64)]),
+        "clf_server_submit_many": (ctypes.c_int, [vp, ctypes.c_int32, u8pp, sz_p,
+                                                  ctypes.POINTER(ctypes.c_uint64),
+                                                  ctypes.POINTER(ctypes.c_uint64),
+                                                  ctypes.POINTER(ctypes.c_int32)]),
         "clf_server_wait": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(Served)]),
         "clf_server_release": (ctypes.c_int, [vp, ctypes.c_uint64]),
         "clf_server_replay": (ctypes.c_int, [vp, ctypes.c_int32, u8pp, sz_p, ctypes.c_int64,
@@ -212,7 +216,7 @@ EXPORTED = (
     "clf_context_num_cus", "clf_context_synchronize", "clf_event_track_capacity", "clf_batch_create", "clf_batch_destroy",
     "clf_batch_run", "clf_batch_results", "clf_batch_time", "clf_search", "clf_presort_event",
     "clf_fill_candidates", "clf_probe_copy", "clf_batch_profile", "clf_server_default_config",
-    "clf_server_create", "clf_server_destroy", "clf_server_submit", "clf_server_wait",
+    "clf_server_create", "clf_server_destroy", "clf_server_submit", "clf_server_submit_many", "clf_server_wait",
     "clf_server_release", "clf_server_replay", "clf_tracks_digest", "clf_server_submit_framed",
     "clf_unframe", "clf_format_tracks", "clf_context_device_name",
 )

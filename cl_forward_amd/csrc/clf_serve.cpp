@@ -46,6 +46,7 @@ int64_t now_ns() {
 }
 
 constexpr int kMaxSlots = 8;
+constexpr int32_t kMaxSubmitMany = 64;  // events per clf_server_submit_many call
 constexpr size_t kBlobAlign = 256;
 constexpr int64_t kEventBlobOverhead = 4096;  // header allowance per event in a slot
 constexpr int64_t kEventCapOverhead = 64 * 48 + 64;
@@ -788,6 +789,143 @@ clf_status create_batches(clf_server* s) {
     return CLF_OK;
 }
 
+// Submits events[0..n) in order and stops at the first one that is
+// malformed or does not fit: `*accepted` of them are queued, and the status
+// is that of the first one refused (CLF_OK if none was).  The requests, arena
+// spans and descriptors of all of them are taken under one acquisition of
+// the server lock; the blobs are copied and the descriptors published after
+// it, in order.
+clf_status submit_events(clf_server* s, int32_t n, const uint8_t* const* events, const size_t* sizes,
+                         const uint64_t* tags, uint64_t* tickets, int32_t* accepted) {
+    struct Sized {
+        int32_t S, N;
+        size_t bytes, span, out_span;
+        int64_t cap, out_cap;
+    };
+    Sized z[kMaxSubmitMany];
+    Request* rq[kMaxSubmitMany];
+    *accepted = 0;
+    clf_status refused = CLF_OK;
+    const char* why = nullptr;
+    int32_t valid = 0;
+    for (; valid < n; ++valid) {
+        Sized& e = z[valid];
+        if (!validate_event(events[valid], sizes[valid], e.S, e.N)) {
+            refused = CLF_E_INVALID_ARGUMENT, why = "malformed event";
+            break;
+        }
+        e.bytes = event_bytes(e.S, e.N);
+        e.span = align_up(e.bytes, kBlobAlign);
+        e.cap = CLF_EVENT_TRACK_CAPACITY(e.N, e.S);
+        e.out_cap = std::min<int64_t>(e.cap, (int64_t)s->conf.out_min_tracks +
+                                                 (int64_t)e.N * s->conf.out_tracks_per_1k_hits / 1000);
+        e.out_span = align_up(sizeof(clf_track) * (size_t)std::max<int64_t>(e.out_cap, 1), kBlobAlign);
+        if (s->resident) {
+            if (e.N > s->wg_hits || e.cap > s->wg_cap || (int64_t)e.span > s->wg_blob_bytes) {
+                refused = CLF_E_CAPACITY, why = "event larger than the serving kernel's scratch";
+                break;
+            }
+        } else if (e.span > s->slot_blob_bytes || e.N > s->conf.max_batch_hits || e.cap > s->slot_cap) {
+            refused = CLF_E_CAPACITY, why = "event larger than a serving batch";
+            break;
+        }
+    }
+    int32_t taken_n = 0;
+    uint64_t d0 = 0;
+    std::string err_msg;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        d0 = s->next_desc;
+        for (; taken_n < valid; ++taken_n) {
+            const Sized& e = z[taken_n];
+            if (s->error != CLF_OK) {
+                refused = s->error, err_msg = s->error_msg;
+                break;
+            }
+            if (s->stop) {
+                refused = CLF_E_INVALID_ARGUMENT, why = "server stopped";
+                break;
+            }
+            if (s->free_reqs.empty()) {
+                refused = CLF_E_CAPACITY, why = "request table full";
+                break;
+            }
+            size_t off, taken, ooff = 0, otaken = 0;
+            if (!s->in_ring.fits(e.span, off, taken)) {
+                refused = CLF_E_CAPACITY, why = "input arena full";
+                break;
+            }
+            uint64_t d = 0;
+            if (s->resident) {
+                d = s->next_desc;
+                if (s->desc_busy[d & s->ring_mask]) {
+                    refused = CLF_E_CAPACITY, why = "descriptor ring full";
+                    break;
+                }
+                if (!s->out_ring.fits(e.out_span, ooff, otaken)) {
+                    refused = CLF_E_CAPACITY, why = "output arena full";
+                    break;
+                }
+            }
+            const int32_t idx = s->free_reqs.back();
+            s->free_reqs.pop_back();
+            Request* r = &s->reqs[idx];
+            r->ticket = (s->next_seq++ << s->req_shift) | (uint64_t)idx;
+            r->tag = tags[taken_n];
+            r->in_off = off;
+            r->span = e.span;
+            r->in_seq = s->in_ring.commit(off, e.span, taken);
+            r->in_live = true;
+            if (s->resident) {
+                r->out_off = ooff;
+                r->out_seq = s->out_ring.commit(ooff, e.out_span, otaken);
+                r->desc = d;
+                s->desc_busy[d & s->ring_mask] = 1;
+                ++s->next_desc;
+            }
+            r->S = e.S;
+            r->N = e.N;
+            r->cap = e.cap;
+            r->out_cap = e.out_cap;
+            r->out_buf = -1;
+            r->out = clf_served{};
+            r->state.store(R_WRITING);
+            if (!s->resident) s->pending.push_back(idx);
+            tickets[taken_n] = r->ticket;
+            rq[taken_n] = r;
+        }
+    }
+    int32_t published = 0;
+    for (int32_t k = 0; k < taken_n; ++k) {
+        Request* r = rq[k];
+        const Sized& e = z[k];
+        std::memcpy(s->arena.as<uint8_t>() + r->in_off, events[k], e.bytes);
+        r->cost = event_cost(events[k], e.S, e.N);
+        r->out.t_submit_ns = now_ns();
+        // a failure of the server may have ended the request meanwhile; its
+        // descriptor is then never read (the kernel has been told to stop)
+        int expected = R_WRITING;
+        if (!r->state.compare_exchange_strong(expected, R_PENDING, std::memory_order_acq_rel)) continue;
+        if (s->resident) {
+            const uint64_t d = d0 + (uint64_t)k;
+            ServeDesc* sd = s->ring_mem.as<ServeDesc>() + (d & s->ring_mask);
+            sd->in_off = r->in_off;
+            sd->out_off = r->out_off;
+            sd->n_hits = e.N;
+            sd->n_sensors = e.S;
+            sd->req = (int32_t)(r - s->reqs.data());
+            sd->out_cap = (int32_t)e.out_cap;
+            __atomic_store_n(&sd->seq, (uint32_t)(d + 1), __ATOMIC_RELEASE);
+            ++published;
+        }
+    }
+    if (published > 0) s->published.fetch_add((uint64_t)published);
+    *accepted = taken_n;
+    if (taken_n > 0) s->cv_work.notify_one();
+    if (refused != CLF_OK) return why ? fail(refused, why) : fail(refused, err_msg);
+    return CLF_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -918,82 +1056,19 @@ clf_status clf_server_submit(clf_server* s, const uint8_t* event, size_t size, u
                              uint64_t* ticket) {
     if (!s || !ticket) return fail(CLF_E_INVALID_ARGUMENT, "null argument");
     *ticket = 0;
-    int32_t S = 0, N = 0;
-    if (!validate_event(event, size, S, N)) return fail(CLF_E_INVALID_ARGUMENT, "malformed event");
-    const size_t bytes = event_bytes(S, N);
-    const size_t span = align_up(bytes, kBlobAlign);
-    const int64_t cap = CLF_EVENT_TRACK_CAPACITY(N, S);
-    const int64_t out_cap =
-        std::min<int64_t>(cap, (int64_t)s->conf.out_min_tracks +
-                                   (int64_t)N * s->conf.out_tracks_per_1k_hits / 1000);
-    const size_t out_span = align_up(sizeof(clf_track) * (size_t)std::max<int64_t>(out_cap, 1), kBlobAlign);
-    if (s->resident) {
-        if (N > s->wg_hits || cap > s->wg_cap || (int64_t)span > s->wg_blob_bytes)
-            return fail(CLF_E_CAPACITY, "event larger than the serving kernel's scratch");
-    } else if (span > s->slot_blob_bytes || N > s->conf.max_batch_hits || cap > s->slot_cap) {
-        return fail(CLF_E_CAPACITY, "event larger than a serving batch");
-    }
-    Request* r = nullptr;
-    uint64_t d = 0;
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        if (s->error != CLF_OK) return fail(s->error, s->error_msg);
-        if (s->stop) return fail(CLF_E_INVALID_ARGUMENT, "server stopped");
-        if (s->free_reqs.empty()) return fail(CLF_E_CAPACITY, "request table full");
-        size_t off, taken, ooff = 0, otaken = 0;
-        if (!s->in_ring.fits(span, off, taken)) return fail(CLF_E_CAPACITY, "input arena full");
-        if (s->resident) {
-            d = s->next_desc;
-            if (s->desc_busy[d & s->ring_mask]) return fail(CLF_E_CAPACITY, "descriptor ring full");
-            if (!s->out_ring.fits(out_span, ooff, otaken))
-                return fail(CLF_E_CAPACITY, "output arena full");
-        }
-        const int32_t idx = s->free_reqs.back();
-        s->free_reqs.pop_back();
-        r = &s->reqs[idx];
-        r->ticket = (s->next_seq++ << s->req_shift) | (uint64_t)idx;
-        r->tag = tag;
-        r->in_off = off;
-        r->span = span;
-        r->in_seq = s->in_ring.commit(off, span, taken);
-        r->in_live = true;
-        if (s->resident) {
-            r->out_off = ooff;
-            r->out_seq = s->out_ring.commit(ooff, out_span, otaken);
-            r->desc = d;
-            s->desc_busy[d & s->ring_mask] = 1;
-            ++s->next_desc;
-        }
-        r->S = S;
-        r->N = N;
-        r->cap = cap;
-        r->out_cap = out_cap;
-        r->out_buf = -1;
-        r->out = clf_served{};
-        r->state.store(R_WRITING);
-        if (!s->resident) s->pending.push_back(idx);
-        *ticket = r->ticket;
-    }
-    std::memcpy(s->arena.as<uint8_t>() + r->in_off, event, bytes);
-    r->cost = event_cost(event, S, N);
-    r->out.t_submit_ns = now_ns();
-    // a failure of the server may have ended the request meanwhile
-    int expected = R_WRITING;
-    if (!r->state.compare_exchange_strong(expected, R_PENDING, std::memory_order_acq_rel))
-        return CLF_OK;
-    if (s->resident) {
-        ServeDesc* sd = s->ring_mem.as<ServeDesc>() + (d & s->ring_mask);
-        sd->in_off = r->in_off;
-        sd->out_off = r->out_off;
-        sd->n_hits = N;
-        sd->n_sensors = S;
-        sd->req = (int32_t)(r - s->reqs.data());
-        sd->out_cap = (int32_t)out_cap;
-        __atomic_store_n(&sd->seq, (uint32_t)(d + 1), __ATOMIC_RELEASE);
-        s->published.fetch_add(1);
-    }
-    s->cv_work.notify_one();
-    return CLF_OK;
+    int32_t accepted = 0;
+    return submit_events(s, 1, &event, &size, &tag, ticket, &accepted);
+}
+
+clf_status clf_server_submit_many(clf_server* s, int32_t n, const uint8_t* const* events,
+                                  const size_t* sizes, const uint64_t* tags, uint64_t* tickets,
+                                  int32_t* n_accepted) {
+    if (n_accepted) *n_accepted = 0;
+    if (!s || !n_accepted || n < 0 || n > kMaxSubmitMany ||
+        (n > 0 && (!events || !sizes || !tags || !tickets)))
+        return fail(CLF_E_INVALID_ARGUMENT, "bad submit_many argument");
+    for (int32_t k = 0; k < n; ++k) tickets[k] = 0;
+    return submit_events(s, n, events, sizes, tags, tickets, n_accepted);
 }
 
 clf_status clf_server_submit_framed(clf_server* s, const uint8_t* frame, size_t size, uint64_t tag,
@@ -1094,29 +1169,46 @@ clf_status clf_server_replay(clf_server* s, int32_t n_pool, const uint8_t* const
     std::atomic<int64_t> max_lag{0};
     std::atomic<int64_t> last_done{start};
 
+    // Thread t submits arrivals t, t + threads, ...; when it runs late, the
+    // arrivals of its share already due go in one clf_server_submit_many.
     auto submitter = [&](int t) {
-        for (int64_t i = t; i < n; i += submit_threads) {
+        const uint8_t* ev[kMaxSubmitMany];
+        size_t sz[kMaxSubmitMany];
+        uint64_t tg[kMaxSubmitMany], tk[kMaxSubmitMany];
+        for (int64_t i = t; i < n;) {
             const int64_t target = start + arrival_ns[i];
             int64_t now = now_ns();
             if (target - now > 120000)  // sleep most of a long gap, spin the rest
                 std::this_thread::sleep_for(std::chrono::nanoseconds(target - now - 80000));
             while ((now = now_ns()) < target) {
             }
-            const int64_t lag = now - target;
-            int64_t m = max_lag.load();
-            while (lag > m && !max_lag.compare_exchange_weak(m, lag)) {
+            int32_t m = 0;
+            for (int64_t j = i; j < n && m < kMaxSubmitMany && (m == 0 || start + arrival_ns[j] <= now);
+                 j += submit_threads) {
+                const int64_t lag = now - (start + arrival_ns[j]);
+                int64_t ml = max_lag.load();
+                while (lag > ml && !max_lag.compare_exchange_weak(ml, lag)) {
+                }
+                const int p = pick[j];
+                ev[m] = pool[p];
+                sz[m] = sizes[p];
+                tg[m] = (uint64_t)j;
+                ++m;
             }
-            uint64_t ticket = 0;
-            const int p = pick[i];
-            clf_status st = clf_server_submit(s, pool[p], sizes[p], (uint64_t)i, &ticket);
-            submit_ns.fetch_add(now_ns() - now);
-            if (st != CLF_OK) {
-                rec[i].status = st;
+            i += (int64_t)m * submit_threads;
+            for (int32_t k = 0; k < m;) {
+                int32_t acc = 0;
+                const clf_status st = clf_server_submit_many(s, m - k, ev + k, sz + k, tg + k, tk + k, &acc);
+                k += acc;
+                if (st == CLF_OK) break;
+                rec[tg[k]].status = st;  // arrival tg[k] was refused; the rest are tried again
+                ++k;
                 if (remaining.fetch_sub(1) == 1) {
                     std::lock_guard<std::mutex> lk(s->mu);
                     s->cv_completed.notify_all();
                 }
             }
+            submit_ns.fetch_add(now_ns() - now);
         }
     };
     auto collector = [&]() {

@@ -311,6 +311,33 @@ class Server:
                    "clf_server_submit")
         return int(t.value)
 
+    def submit_many(self, events, tags=None):
+        """Queues events in order under one acquisition of the server lock
+        (clf_server_submit_many, at most 64 a call).  Returns the tickets
+        of the events accepted; raises the first refusal's error if any
+        event was refused (the ones before it stay queued: their tickets
+        are in the exception's `tickets` attribute)."""
+        bs = [bytes(e) for e in events]
+        n = len(bs)
+        tags = list(range(n)) if tags is None else [int(x) for x in tags]
+        if len(tags) != n:
+            raise ValueError("submit_many: one tag per event")
+        bufs = [(ctypes.c_uint8 * max(len(b), 1)).from_buffer_copy(b or b"\0") for b in bs]
+        u8p = ctypes.POINTER(ctypes.c_uint8)
+        ptrs = (u8p * max(n, 1))(*[ctypes.cast(b, u8p) for b in bufs])
+        sizes = (ctypes.c_size_t * max(n, 1))(*[len(b) for b in bs])
+        tg = (ctypes.c_uint64 * max(n, 1))(*tags)
+        tk = (ctypes.c_uint64 * max(n, 1))()
+        acc = ctypes.c_int32()
+        st = self._L.clf_server_submit_many(self._h, n, ptrs, sizes, tg, tk, ctypes.byref(acc))
+        tickets = [int(tk[k]) for k in range(acc.value)]
+        try:
+            _lib.check(st, "clf_server_submit_many")
+        except Exception as e:
+            e.tickets = tickets
+            raise
+        return tickets
+
     def submit_framed(self, frame: bytes, tag: int = 0) -> int:
         """submit() of a request in the Gaudi offload framing."""
         b = bytes(frame)

@@ -242,6 +242,16 @@ void clf_server_destroy(clf_server* server);
  * rejected with CLF_E_INVALID_ARGUMENT. */
 clf_status clf_server_submit(clf_server* server, const uint8_t* event, size_t size, uint64_t tag,
                              uint64_t* ticket);
+/* Submits events[0..n) (n <= 64) in order, as clf_server_submit would one by
+ * one, but takes the server's lock once for all of them; stops at the first
+ * event refused.  *n_accepted events are queued (tickets[k] set for those,
+ * 0 for the rest) and the status is that of the first refused (CLF_OK if
+ * none was).  For a front end that receives several requests at once, or
+ * has fallen behind.  No counterpart in the reference, which has no serving
+ * front end. */
+clf_status clf_server_submit_many(clf_server* server, int32_t n, const uint8_t* const* events,
+                                  const size_t* sizes, const uint64_t* tags, uint64_t* tickets,
+                                  int32_t* n_accepted);
 /* Waits up to timeout_ms (< 0: no limit) for a request; CLF_E_CAPACITY on
  * time-out (the request stays queued). */
 clf_status clf_server_wait(clf_server* server, uint64_t ticket, int32_t timeout_ms,
@@ -283,7 +293,9 @@ typedef struct clf_replay_summary {
      * input copy over the host link, search, track copy and completion */
     double dev_copy_in_us, dev_search_us, dev_copy_out_us;
     double dev_busy_frac;     /* workgroup time with an event / (workers x wall) */
-    double host_submit_us;    /* mean duration of clf_server_submit             */
+    double host_submit_us;    /* mean submit time per arrival (a late submitter
+                                 queues the arrivals already due together, with
+                                 clf_server_submit_many)                        */
     double host_digest_us;    /* mean time to digest and release one result     */
 } clf_replay_summary;
 

@@ -63,6 +63,38 @@ def test_serve_mix_equals_alone(gpu_ctx, plain51_blobs, plain_ref, mcdata_sorted
 
 
 @ENGINES
+def test_serve_submit_many(gpu_ctx, plain51_blobs, plain_ref, mcdata_sorted, engine):
+    """clf_server_submit_many queues a whole group in one call with the same
+    results as one submit each, and stops at the first event refused: the
+    ones before it stay queued and finish exactly."""
+    ref_outs, _ = plain_ref
+    mc = [mcdata_sorted[k] for k in list(mcdata_sorted)[:12]]
+    mc_ref, _ = gpu_ctx.search(mc, MODE_PLAIN)
+    events = plain51_blobs + mc
+    want = ref_outs + mc_ref
+    with Server(gpu_ctx, engine=engine, slots=3, max_batch_events=32, max_batch_hits=1 << 18,
+                max_wait_us=200) as srv:
+        tickets = srv.submit_many(events[:40], tags=range(40)) + srv.submit_many(events[40:])
+        assert len(tickets) == len(events) and 0 not in tickets
+        for i, t in enumerate(tickets):
+            assert srv.wait(t, timeout_ms=30000)[0] == want[i], f"event {i} differs"
+        assert srv.submit_many([]) == []
+        with pytest.raises(ClfError) as e:  # malformed second event
+            srv.submit_many([events[3], events[4][:100], events[5]])
+        assert e.value.status == 1 and len(e.value.tickets) == 1
+        assert srv.wait(e.value.tickets[0], timeout_ms=30000)[0] == want[3]
+    small = line_event([(1.0, 2.0, 0.01, -0.02), (-3.0, 1.0, -0.03, 0.01)],
+                       [-300 + 22 * k for k in range(20)]).pack()
+    with Server(gpu_ctx, engine=engine, slots=1, max_batch_events=4, max_batch_hits=1000,
+                max_event_hits=1000, max_requests=2) as srv:
+        with pytest.raises(ClfError) as e:  # request table full after two
+            srv.submit_many([small, small, small])
+        assert e.value.status == 5 and len(e.value.tickets) == 2
+        a, b = (srv.wait(t, timeout_ms=20000)[0] for t in e.value.tickets)
+        assert a == b
+
+
+@ENGINES
 def test_serve_replay_digests(gpu_ctx, plain51_blobs, plain_ref, engine):
     """Open-loop replay of 4000 arrivals drawn from the 51 events; every
     result's digest equals the plain path's."""
