@@ -256,6 +256,13 @@ def main(argv=None):
     flags_any = 0
     for s in stats:
         flags_any |= s["flags"]
+    # the residual against the reference, enumerated: tracks whose 24-slot
+    # record was full when they found a 25th hit (the reference appends it
+    # past the record there; here the hit is marked used and the track stops)
+    overflow = {"tracks": sum(s["n_overflow"] for s in stats),
+                "events": sum(1 for s in stats if s["n_overflow"] > 0),
+                "full_records": sum(s["n_full"] for s in stats),
+                "per_event_max": max((s["n_overflow"] for s in stats), default=0)}
 
     # timed region: exactly `steps` graph replays, synchronised on both sides
     for _ in range(a.warmup):
@@ -374,6 +381,8 @@ def main(argv=None):
         "batched_vs_plain": round(value / plain["events_per_s"], 2) if plain else None,
         "end_to_end": e2e,
         "oracle_identical_events": oracle_identical,
+        "overflow_tracks": overflow["tracks"],
+        "overflow": overflow,
         "roofline": roofline,
         "valu_roofline": valu,
         "cpu_baseline": cpu,

@@ -23,9 +23,9 @@ class EventStats(ctypes.Structure):
         ("n_tracklets", ctypes.c_int32),
         ("n_weak", ctypes.c_int32),
         ("max_ttf_live", ctypes.c_int32),
-        ("n_frozen", ctypes.c_int32),
+        ("n_full", ctypes.c_int32),
         ("h0_visits", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("n_overflow", ctypes.c_int32),
     ]
 
 
@@ -175,6 +175,10 @@ def load():
                                                ctypes.POINTER(ctypes.POINTER(ctypes.c_int32)),
                                                ctypes.POINTER(ctypes.POINTER(ctypes.c_int32)),
                                                ctypes.POINTER(ctypes.c_uint32)]),
+        "clf_fill_candidates_search_path": (ctypes.c_int, [vp, ctypes.c_int32, u8pp, sz_p,
+                                                           ctypes.POINTER(ctypes.POINTER(ctypes.c_int32)),
+                                                           ctypes.POINTER(ctypes.POINTER(ctypes.c_int32)),
+                                                           ctypes.POINTER(ctypes.c_uint32)]),
         "clf_server_default_config": (None, [ctypes.POINTER(ServerConfig)]),
         "clf_server_create": (ctypes.c_int, [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]),
         "clf_server_destroy": (None, [vp]),
@@ -215,7 +219,7 @@ EXPORTED = (
     "clf_context_destroy", "clf_context_set_config", "clf_context_get_config",
     "clf_context_num_cus", "clf_context_synchronize", "clf_event_track_capacity", "clf_batch_create", "clf_batch_destroy",
     "clf_batch_run", "clf_batch_results", "clf_batch_time", "clf_search", "clf_presort_event",
-    "clf_fill_candidates", "clf_probe_copy", "clf_batch_profile", "clf_server_default_config",
+    "clf_fill_candidates", "clf_fill_candidates_search_path", "clf_probe_copy", "clf_batch_profile", "clf_server_default_config",
     "clf_server_create", "clf_server_destroy", "clf_server_submit", "clf_server_submit_many", "clf_server_wait",
     "clf_server_release", "clf_server_replay", "clf_tracks_digest", "clf_server_submit_framed",
     "clf_unframe", "clf_format_tracks", "clf_context_device_name",

@@ -608,9 +608,15 @@ clf_status clf_probe_copy(clf_context* ctx, int64_t bytes, int32_t reps, float* 
     return CLF_OK;
 }
 
-clf_status clf_fill_candidates(clf_context* ctx, int32_t n, const uint8_t* const* events,
-                               const size_t* sizes, int32_t* const* cand_h1,
-                               int32_t* const* cand_h2, uint32_t* flags) {
+}  // extern "C"
+
+namespace {
+
+// Windows of every hit through fill mode `mode` of the search kernel (1:
+// binary searches over the blob in HBM; 2: the search's LDS ring path).
+clf_status fill_windows(clf_context* ctx, int32_t n, const uint8_t* const* events,
+                        const size_t* sizes, int32_t* const* cand_h1, int32_t* const* cand_h2,
+                        uint32_t* flags, int mode) {
     if (!ctx || n < 0 || (n > 0 && (!cand_h1 || !cand_h2)))
         return fail(CLF_E_INVALID_ARGUMENT, "null argument");
     clf_batch* b = nullptr;
@@ -624,7 +630,7 @@ clf_status clf_fill_candidates(clf_context* ctx, int32_t n, const uint8_t* const
     hipStream_t st = ctx->stream;
     HIP_TRY(launch_zero_i32(b->dev.queue.as<int32_t>(), n + 1, st));
     SearchArgs a = b->dev.args(n);
-    a.fill_only = 1;
+    a.fill_only = mode;
     HIP_TRY(launch_search(a, ctx->cfg.block_size, ctx->cfg.lds_hits, resolved_grid(ctx->cfg, ctx->n_cu, n), st));
     std::vector<int4> cand((size_t)std::max<int64_t>(b->total_hits, 1));
     std::vector<EventResult> res(n);
@@ -647,6 +653,23 @@ clf_status clf_fill_candidates(clf_context* ctx, int32_t n, const uint8_t* const
         }
     }
     return CLF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+clf_status clf_fill_candidates(clf_context* ctx, int32_t n, const uint8_t* const* events,
+                               const size_t* sizes, int32_t* const* cand_h1,
+                               int32_t* const* cand_h2, uint32_t* flags) {
+    return fill_windows(ctx, n, events, sizes, cand_h1, cand_h2, flags, 1);
+}
+
+clf_status clf_fill_candidates_search_path(clf_context* ctx, int32_t n,
+                                           const uint8_t* const* events, const size_t* sizes,
+                                           int32_t* const* cand_h1, int32_t* const* cand_h2,
+                                           uint32_t* flags) {
+    return fill_windows(ctx, n, events, sizes, cand_h1, cand_h2, flags, 2);
 }
 
 clf_status clf_search(clf_context* ctx, clf_mode mode, int32_t n, const uint8_t* const* events,

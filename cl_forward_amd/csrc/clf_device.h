@@ -24,9 +24,9 @@ struct EventResult {
     int32_t n_tracklets;
     int32_t n_weak;
     int32_t max_ttf_live;
-    int32_t n_frozen;
+    int32_t n_full;      // tracks that reached 24 hits
     int32_t h0_visits;
-    int32_t reserved;
+    int32_t n_overflow;  // 24-hit tracks that found a 25th hit (CLF_FLAG_TRACK_FULL)
 };
 
 // A track or tracklet to follow.  "word" is the reference's ttf word

@@ -215,7 +215,8 @@ struct SharedCore {
     int32_t item;
     uint32_t flags;
     int32_t max_live;
-    int32_t n_frozen;
+    int32_t n_full;
+    int32_t n_overflow;
     // gather state of the current sensor step (Kernel.cl:720-721,753-754)
     uint32_t g_prev, g_shift, g_counter;
     int32_t g_nv, g_more;
@@ -1244,7 +1245,10 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
                 if (bk >= 0) {
                     found = true;
                     f_new = is_tl;
-                    f_req = t.nhits + 1 < CLF_MAX_TRACK_SIZE;
+                    // re-queued (Kernel.cl:343-345), unless the record is
+                    // full: its 25th hit would be appended past the record
+                    // (:321); the hit is still marked used (:316) below
+                    f_req = t.nhits < CLF_MAX_TRACK_SIZE;
                 } else {
                     // NaN best with no hit recorded: only in the
                     // degenerate domain; the track is dropped
@@ -1270,7 +1274,11 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
         int pos[3], tot[3];
         block_scan3<B, C>(sh, f_new, f_req, f_weak, pos, tot, st.sel);
         if (active) {
-            if (found) {
+            if (found && !f_req) {
+                // a full record found a 25th hit: the record stays as stored
+                atomicOr(&sh.flags, CLF_FLAG_TRACK_FULL);
+                atomicAdd(&sh.n_overflow, 1);
+            } else if (found) {
                 const int b = v4.start + bk;
                 uint32_t slot;
                 if (f_new) {
@@ -1288,23 +1296,19 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
                     tracks[slot].hits[t.nhits] = (uint32_t)b;
                     tracks[slot].hitsNum = t.nhits + 1;
                 }
-                if (f_req) {
-                    TtfEntry n;
-                    n.word = slot;
-                    n.nhits = t.nhits + 1;
-                    n.t0 = n.t1 = n.t2 = 0;
-                    n.x0 = t.x1;
-                    n.y0 = t.y1;
-                    n.z0 = t.z1;
-                    n.x1 = v4.x[bk];
-                    n.y1 = v4.y[bk];
-                    n.z1 = v4.z[bk];
-                    n.pad = 0;
-                    ttf_put(sh, nxt, n_next + pos[1], n);
-                } else {
-                    atomicOr(&sh.flags, CLF_FLAG_TRACK_FULL);
-                    atomicAdd(&sh.n_frozen, 1);
-                }
+                if (t.nhits + 1 == CLF_MAX_TRACK_SIZE) atomicAdd(&sh.n_full, 1);
+                TtfEntry n;
+                n.word = slot;
+                n.nhits = t.nhits + 1;
+                n.t0 = n.t1 = n.t2 = 0;
+                n.x0 = t.x1;
+                n.y0 = t.y1;
+                n.z0 = t.z1;
+                n.x1 = v4.x[bk];
+                n.y1 = v4.y[bk];
+                n.z1 = v4.z[bk];
+                n.pad = 0;
+                ttf_put(sh, nxt, n_next + pos[1], n);
             } else if (f_req) {
                 const uint32_t skipped = (t.word & CLF_TTF_SKIP_MASK) >> CLF_TTF_SKIP_SHIFT;
                 TtfEntry n = t;
@@ -1450,7 +1454,8 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
     if (tid == 0) {
         sh.flags = 0;
         sh.max_live = 0;
-        sh.n_frozen = 0;
+        sh.n_full = 0;
+        sh.n_overflow = 0;
         sh.n_listed = 0;
     }
     __syncthreads();
@@ -1502,22 +1507,46 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
     const bool degenerate = (sh.flags & CLF_FLAG_DEGENERATE_Z) != 0;
 
     if (a.fill_only) {
-        // stage (a) alone, from HBM: every hit's windows in the reference's
-        // absolute-index form (dev_hit_candidates / dev_hit_h2_candidates)
+        // stage (a) alone: every hit's windows in the reference's
+        // absolute-index form (dev_hit_candidates / dev_hit_h2_candidates).
+        // fill_only 1: binary searches over the event blob in HBM;
+        // fill_only 2: the search's own path, both sensors of a window
+        // staged into their ring slots with their x-bucket index by the
+        // staging code of the configuration (sensors above C hits stay in
+        // HBM), then the same window searches.
         int4* cand = a.cand + d.hit_off;
         for (int i = tid; i < N; i += B) cand[i] = make_int4(-1, -1, -1, -1);
         __syncthreads();
         for (int f = 4; f < S; ++f) {
-            const SView v2 = hbm_view(sh, ev, f - 2);
-            const int st = sh.s_start[f], n = sh.s_num[f];
+            SView v2, v0;
+            if (a.fill_only == 2) {
+                __syncthreads();  // the ring slots' previous readers are done
+                if constexpr (B <= 512) {
+                    StagedSensor<B, C> g;
+                    stage_load<B, C>(sh, ev, f, g);
+                    stage_store<B, C>(sh, g);
+                    stage_load<B, C>(sh, ev, f - 2, g);
+                    stage_store<B, C>(sh, g);
+                } else {
+                    stage_sensor<B, C>(sh, ev, f);
+                    stage_sensor<B, C>(sh, ev, f - 2);
+                }
+                __syncthreads();
+                v2 = sensor_view<false>(sh, ev, f - 2);
+                v0 = sensor_view<false>(sh, ev, f);
+            } else {
+                v2 = hbm_view(sh, ev, f - 2);
+                v0 = hbm_view(sh, ev, f);
+            }
             const bool p2 = f <= S - 3;
-            for (int i = tid; i < n; i += B) {
-                const float x0 = ev.X[st + i], z0 = ev.Zh[st + i];
+            for (int i = tid; i < v0.n; i += B) {
+                const float x0 = v0.x[i], z0 = v0.z[i];
                 const int2 w1 = window_h1(v2, x0, z0);
                 int2 w2 = make_int2(-1, -1);
                 if (p2) w2 = window_h2(v2, x0, z0, sh.s_Z[f + 2], sh.s_Z[f - 2]);
-                cand[st + i] = make_int4(w1.x < 0 ? -1 : v2.start + w1.x, w1.x < 0 ? -1 : v2.start + w1.y,
-                                         w2.x < 0 ? -1 : v2.start + w2.x, w2.x < 0 ? -1 : v2.start + w2.y);
+                cand[v0.start + i] =
+                    make_int4(w1.x < 0 ? -1 : v2.start + w1.x, w1.x < 0 ? -1 : v2.start + w1.y,
+                              w2.x < 0 ? -1 : v2.start + w2.x, w2.x < 0 ? -1 : v2.start + w2.y);
             }
         }
         if (tid == 0) {
@@ -1639,9 +1668,9 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
         r.n_tracklets = n_tracklets;
         r.n_weak = n_weak;
         r.max_ttf_live = sh.max_live;
-        r.n_frozen = sh.n_frozen;
+        r.n_full = sh.n_full;
         r.h0_visits = visits;
-        r.reserved = 0;
+        r.n_overflow = sh.n_overflow;
         *res = r;
     }
     prof_mark(sh, pc, PROF_TAIL);

@@ -125,9 +125,10 @@ class Context:
             b.run(mode)
             return b.results()
 
-    def fill_candidates(self, events):
+    def fill_candidates(self, events, search_path: bool = False):
         """Stage (a) alone.  Returns a list of (cand_h1, cand_h2, flags) per event,
-        the windows as int32 numpy arrays of 2*N."""
+        the windows as int32 numpy arrays of 2*N.  search_path: computed as
+        the search computes them (LDS ring slots, x-bucket index)."""
         import numpy as np
 
         n = len(events)
@@ -142,8 +143,8 @@ class Context:
         p1 = (i32p * max(n, 1))(*[a[0].ctypes.data_as(i32p) for a in arrs])
         p2 = (i32p * max(n, 1))(*[a[1].ctypes.data_as(i32p) for a in arrs])
         flags = (ctypes.c_uint32 * max(n, 1))()
-        _lib.check(self._L.clf_fill_candidates(self._h, n, ptrs, sizes, p1, p2, flags),
-                   "clf_fill_candidates")
+        fn = self._L.clf_fill_candidates_search_path if search_path else self._L.clf_fill_candidates
+        _lib.check(fn(self._h, n, ptrs, sizes, p1, p2, flags), "clf_fill_candidates")
         del holders
         return [(a[0][: 2 * a[2]], a[1][: 2 * a[2]], int(flags[i])) for i, a in enumerate(arrs)]
 
@@ -194,7 +195,7 @@ class Batch:
         stats = []
         for i in range(n):
             s = r.stats[i]
-            stats.append({k: int(getattr(s, k)) for k, _ in EventStats._fields_ if k != "reserved"})
+            stats.append({k: int(getattr(s, k)) for k, _ in EventStats._fields_})
         return outs, stats
 
     PROFILE_FIELDS = ("prologue", "stage", "forward", "gather", "seed_setup", "stage_wait",
@@ -356,7 +357,7 @@ class Server:
         try:
             _lib.check(o.status, "served request")
             raw = ctypes.string_at(o.tracks, o.n_tracks * TRACK_BYTES) if o.n_tracks else b""
-            stats = {k: int(getattr(o.stats, k)) for k, _ in EventStats._fields_ if k != "reserved"}
+            stats = {k: int(getattr(o.stats, k)) for k, _ in EventStats._fields_}
             timing = {"submit_ns": int(o.t_submit_ns), "launch_ns": int(o.t_launch_ns),
                       "done_ns": int(o.t_done_ns), "batch_events": int(o.batch_events)}
         finally:

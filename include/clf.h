@@ -53,9 +53,12 @@ typedef struct clf_event_stats {
     int32_t n_tracklets;  /* tracklets made by seeding                     */
     int32_t n_weak;       /* weak entries                                  */
     int32_t max_ttf_live; /* largest ttf ring span over the sensor steps   */
-    int32_t n_frozen;     /* tracks frozen at 24 hits                      */
+    int32_t n_full;       /* tracks that reached 24 hits (a full record)   */
     int32_t h0_visits;    /* h0 visited by the gather, repeats included    */
-    int32_t reserved;
+    int32_t n_overflow;   /* 24-hit tracks that found a 25th hit: the hit
+                             is marked used and the track stops there, where
+                             the reference writes past its record
+                             (CLF_FLAG_TRACK_FULL)                         */
 } clf_event_stats;
 
 /* Launch parameters (no effect on results). */
@@ -145,6 +148,17 @@ clf_status clf_search(clf_context* ctx, clf_mode mode, int32_t n_events,
 clf_status clf_fill_candidates(clf_context* ctx, int32_t n_events, const uint8_t* const* events,
                                const size_t* sizes, int32_t* const* cand_h1,
                                int32_t* const* cand_h2, uint32_t* flags);
+
+/* Verification probe, no counterpart in the reference: the same windows,
+ * computed the way the search computes them under the context's launch
+ * configuration (both sensors of a window staged into LDS ring slots with
+ * their x-bucket index by the configuration's staging code, bucket-started
+ * scans; sensors above lds_hits hits read from HBM).  Same output form as
+ * clf_fill_candidates, which it must equal. */
+clf_status clf_fill_candidates_search_path(clf_context* ctx, int32_t n_events,
+                                           const uint8_t* const* events, const size_t* sizes,
+                                           int32_t* const* cand_h1, int32_t* const* cand_h2,
+                                           uint32_t* flags);
 
 /* Measurement probe, no counterpart in the reference: runs the batch once
  * with per-event phase timers and work counters on.  out receives 16

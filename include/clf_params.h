@@ -67,9 +67,13 @@ typedef struct clf_track {
 #define CLF_EVENT_TRACK_CAPACITY(N, S) ((int64_t)(N) + 64 * (int64_t)(S) + 64)
 
 /* ---- Per-event status flags ------------------------------------------ */
-/* A track reached 24 hits; it is stored and no longer followed.  The
- * reference writes a 25th hit past its private record here (undefined
- * behaviour, Kernel.cl:321), so tracks of such events are not comparable. */
+/* A track that already holds 24 hits found a 25th.  The reference marks
+ * that hit used (Kernel.cl:316, defined) and then appends it past its
+ * private 24-slot record (Kernel.cl:321, undefined behaviour).  Here the
+ * mark is made, the 24-hit record stays as stored and the track is not
+ * followed further; each such track is counted (n_overflow).  Everything
+ * the reference does before that append is reproduced: a 24-hit track is
+ * re-queued and followed, skips included, like any other.               */
 #define CLF_FLAG_TRACK_FULL      (1u << 0)
 /* More than MAX_TRACKS output tracks: the reference writes into the next
  * event's slice (Kernel.cl:609,627).                                       */

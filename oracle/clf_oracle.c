@@ -30,8 +30,10 @@
  * orders the reference can produce; the HIP path produces this same order.
  *
  * Policies where the reference is undefined (clf_params.h CLF_FLAG_*):
- *   - a track that reaches 24 hits is stored and no longer followed
- *     (the reference writes past its private Track record);
+ *   - a track that holds 24 hits is re-queued and followed like any other;
+ *     when it finds a hit, that hit is marked used (Kernel.cl:316) and the
+ *     track stops there, where the reference appends a 25th hit past its
+ *     private Track record (Kernel.cl:321); each such stop is counted;
  *   - no capacity of its own: tracks, tracklets, weak and ttf lists are
  *     sized to the event's upper bound and the reference's capacity
  *     overruns are only flagged.
@@ -319,7 +321,15 @@ static void forward_one(state_t* s, uint32_t fulltrackno, int start4, int num4) 
             s->st->flags |= CLF_FLAG_DEGENERATE_Z;
             return;
         }
-        s->used[best_hit_h2] = 1;
+        s->used[best_hit_h2] = 1; /* :316 */
+        if (t.hitsNum >= CLF_MAX_TRACK_SIZE) {
+            /* :321 would append a 25th hit past the 24-slot record
+             * (undefined).  The mark above is the last defined effect: the
+             * stored 24-hit record stays, the track is not re-queued. */
+            s->st->flags |= CLF_FLAG_TRACK_FULL;
+            s->st->n_overflow++;
+            return;
+        }
         t.hits[t.hitsNum++] = best_hit_h2;
         if (t.hitsNum <= 4) {
             s->used[t.hits[0]] = 1;
@@ -330,13 +340,9 @@ static void forward_one(state_t* s, uint32_t fulltrackno, int start4, int num4) 
             trackno = (uint32_t)idx;
         }
         s->tracks[trackno] = t;
-        if (t.hitsNum >= CLF_MAX_TRACK_SIZE) {
-            /* policy: a full record is final */
-            s->st->flags |= CLF_FLAG_TRACK_FULL;
-            s->st->n_frozen++;
-        } else if (ulist_push(&s->ttf, trackno)) {
-            s->oom = 1;
-        }
+        if (t.hitsNum == CLF_MAX_TRACK_SIZE) s->st->n_full++;
+        /* :343-345, a full record included */
+        if (ulist_push(&s->ttf, trackno)) s->oom = 1;
     } else if (skipped <= CLF_MAX_SKIPPED_MODULES) {
         const uint32_t w = ((skipped + 1) << CLF_TTF_SKIP_SHIFT) | (fulltrackno & CLF_TTF_KEEP_MASK);
         if (ulist_push(&s->ttf, w)) s->oom = 1;

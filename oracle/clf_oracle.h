@@ -28,7 +28,9 @@ typedef struct clf_oracle_stats {
     int32_t  n_tracklets;     /* tracklets made by seeding                    */
     int32_t  n_weak;          /* weak entries                                 */
     int32_t  max_ttf_live;    /* largest ring occupancy over the steps        */
-    int32_t  n_frozen;        /* tracks frozen at 24 hits                     */
+    int32_t  n_full;          /* tracks that reached 24 hits                  */
+    int32_t  n_overflow;      /* 24-hit tracks that found a 25th hit          */
+    int32_t  pad;
     int64_t  seed_h0_visits;  /* gathered h0, with the gather's repeats       */
     int64_t  seed_pairs;      /* (h0,h1) pairs evaluated (live, unused h1)    */
     int64_t  seed_triplets;   /* h2 window members evaluated for those pairs  */

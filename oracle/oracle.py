@@ -26,7 +26,9 @@ class Stats(ctypes.Structure):
         ("n_tracklets", ctypes.c_int32),
         ("n_weak", ctypes.c_int32),
         ("max_ttf_live", ctypes.c_int32),
-        ("n_frozen", ctypes.c_int32),
+        ("n_full", ctypes.c_int32),
+        ("n_overflow", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
         ("seed_h0_visits", ctypes.c_int64),
         ("seed_pairs", ctypes.c_int64),
         ("seed_triplets", ctypes.c_int64),
@@ -34,7 +36,7 @@ class Stats(ctypes.Structure):
     ]
 
     def as_dict(self):
-        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+        return {k: int(getattr(self, k)) for k, _ in self._fields_ if k != "pad"}
 
 
 def build(force=False):

@@ -3,7 +3,10 @@
 Run in a container that has the reference tree at /root/reference and the
 oracle built (``make -C oracle all ref``):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [--digests-only]
+
+``--digests-only`` rewrites oracle_digests.json alone, from the committed
+mcdata fixtures (no reference tree needed).
 
 Fixtures (data only):
 
@@ -44,8 +47,27 @@ def sha(b: bytes) -> str:
     return hashlib.sha256(b).hexdigest()
 
 
+def write_digests(srt):
+    digests = {"mcdata": {}, "single": {}, "plain51": {}}
+    for stem, arr in srt.items():
+        out, st = oracle.search_event(arr.tobytes())
+        digests["mcdata"][stem] = {"sha256": sha(out), **st}
+    out, st = oracle.search_event(generator.single_event().pack())
+    digests["single"]["0"] = {"sha256": sha(out), **st}
+    for i in range(51):
+        out, st = oracle.search_event(generator.plain_event(i).pack())
+        digests["plain51"][str(i)] = {"sha256": sha(out), **st}
+    with open(os.path.join(HERE, "oracle_digests.json"), "w") as f:
+        json.dump(digests, f, indent=1, sort_keys=True)
+
+
 def main():
     oracle.build()
+    if "--digests-only" in sys.argv[1:]:
+        z = np.load(os.path.join(HERE, "mcdata_presort_ref.npz"))
+        write_digests({k: z[k] for k in sorted(z.files, key=int)})
+        print("wrote oracle_digests.json")
+        return
     if not oracle.ref_available():
         raise SystemExit("oracle/_ref/libref_tools.so is missing (make -C oracle ref)")
     files = sorted(glob.glob(os.path.join(MCDATA, "*.dat")), key=lambda p: int(os.path.basename(p)[:-4]))
@@ -62,17 +84,7 @@ def main():
     with open(os.path.join(HERE, "mcdata_layout_ref.json"), "w") as f:
         json.dump(layout, f, indent=0, sort_keys=True)
 
-    digests = {"mcdata": {}, "single": {}, "plain51": {}}
-    for stem, arr in srt.items():
-        out, st = oracle.search_event(arr.tobytes())
-        digests["mcdata"][stem] = {"sha256": sha(out), **st}
-    out, st = oracle.search_event(generator.single_event().pack())
-    digests["single"]["0"] = {"sha256": sha(out), **st}
-    for i in range(51):
-        out, st = oracle.search_event(generator.plain_event(i).pack())
-        digests["plain51"][str(i)] = {"sha256": sha(out), **st}
-    with open(os.path.join(HERE, "oracle_digests.json"), "w") as f:
-        json.dump(digests, f, indent=1, sort_keys=True)
+    write_digests(srt)
     print("wrote fixtures for", len(raw), "mcdata events")
 
 

@@ -25,8 +25,9 @@ STAT_PAIRS = (
     ("n_tracklets", "n_tracklets"),
     ("n_weak", "n_weak"),
     ("max_ttf_live", "max_ttf_live"),
-    ("n_frozen", "n_frozen"),
+    ("n_full", "n_full"),
     ("h0_visits", "seed_h0_visits"),
+    ("n_overflow", "n_overflow"),
 )
 
 

@@ -27,7 +27,8 @@ import pytest
 from cl_forward_amd import event as ev_mod
 from cl_forward_amd import generator
 from tests import simt_reference as simt
-from tests.helpers import add_noise, line_event, raw_blob, sub_event
+from tests.helpers import (add_noise, line_event, raw_blob, sparse_line_event, sub_event,
+                           window_pin_events)
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 F = np.float32
@@ -312,6 +313,129 @@ def test_slope_cuts(oracle_mod, slope, seeded):
     ev = line_event([(0.5, 0.3, 0.05, slope)], planes)
     out, _ = oracle_mod.search_event(ev.pack())
     assert (len(ev_mod.track_lists(out)) > 0) == seeded
+
+
+# ---------------------------------------------------------------------------
+# candidate windows worked out by hand (A8, Kernel.cl:50-178)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("case", window_pin_events(), ids=lambda c: c[0])
+def test_windows_hand_worked(oracle_mod, case):
+    """The h2 window from the sensor Zs and a hit z off its sensor's Z, the
+    else-if "last" that keeps one element past xmax, -1 and the sensor end,
+    the h1 slope cut one ulp either side, and the h1 window that ends at the
+    first failing element although later ones pass (see the derivations in
+    tests/helpers.py)."""
+    name, blob, want = case
+    c1, c2 = oracle_mod.fill_candidates(blob)
+    N = len(c1) // 2
+    for h in range(N):
+        w1, w2 = want.get(h, ((-1, -1), (-1, -1)))
+        assert (int(c1[2 * h]), int(c1[2 * h + 1])) == w1, (name, h, "h1")
+        assert (int(c2[2 * h]), int(c2[2 * h + 1])) == w2, (name, h, "h2")
+    assert oracle_mod.input_flags(blob) == 0
+
+
+def test_banded_noise_leaves_the_tracks(oracle_mod):
+    """Noise in x bands far outside the detector, 1000 mm apart for sensors
+    two apart, enters no window and no tolerance: the tracks (as hit-ID
+    sets) of the event with 720 such hits per sensor (37,054 hits, above
+    the 32,768 the kernel keeps LDS used flags for) are those of the event
+    without them."""
+    from tests.helpers import banded_noise_event
+    from tests.parity import id_sets
+
+    base = generator.plain_event(3)
+    big = banded_noise_event(base, 720, 5)
+    assert big.n_hits > 32768
+    b0, b1 = base.pack(), big.pack()
+    assert id_sets(oracle_mod.search_event(b1)[0], b1) == id_sets(oracle_mod.search_event(b0)[0], b0)
+
+
+# ---------------------------------------------------------------------------
+# forwarding skips, the tail and the 24-slot record, worked out by hand
+# ---------------------------------------------------------------------------
+PLANES17 = [22 * k for k in range(17)]
+PLANES48 = [-300 + 22 * k for k in range(48)]
+
+
+def _plane_tracks(blob, out):
+    sensor = ev_mod.parse_event(blob).sensor_of_hit()
+    return [[int(sensor[h]) for h in t] for t in ev_mod.track_lists(out)]
+
+
+@pytest.mark.parametrize("last,want", [(7, [[16, 14, 12, 7]]), (6, [[16, 14, 12]])])
+def test_skip_limit(oracle_mod, last, want):
+    """A tracklet (16, 14, 12) misses sensors 11, 10, 9 and 8 (steps 15-12,
+    skip counts 1-4: a count <= 3 is re-queued, Kernel.cl:349-356).  At step
+    11 it searches sensor 7: a hit there extends it to a track; without one
+    its count 4 exceeds 3 and, holding three hits, it goes to weak
+    (:357-361) and comes out as a 3-hit track (its hits are unused).  A hit
+    on sensor 6 is never searched."""
+    blob = sparse_line_event([(1.0, 2.0, 0.01, 0.02, [16, 14, 12, last])], PLANES17)
+    out, st = oracle_mod.search_event(blob)
+    assert _plane_tracks(blob, out) == want
+    assert st["n_weak"] == (1 if last == 6 else 0)
+
+
+def test_tail_takes_only_the_last_step(oracle_mod):
+    """Line A (13, 11, 9) misses four sensors and goes to weak in the
+    forwarding of step 8.  Line B (8, 6, 4) misses sensors 3 .. 0 and is
+    re-queued at step 4, the last step: the tail turns the tracklet entries
+    inserted in that step, and only those, into weak entries
+    (Kernel.cl:769-792).  Each comes out once, A first (weak order).  A
+    tail over all tracklet entries would repeat A five times and B four."""
+    blob = sparse_line_event([(0.5, 1.0, 0.01, 0.005, [8, 6, 4]),
+                              (60.0, -3.0, 0.02, 0.01, [13, 11, 9])], PLANES17)
+    out, st = oracle_mod.search_event(blob)
+    assert _plane_tracks(blob, out) == [[13, 11, 9], [8, 6, 4]]
+    assert st["n_weak"] == 2 and st["n_tracklets"] == 2
+
+
+def _one_line_48(drop=()):
+    """One noiseless line through 48 planes 22 mm apart (hit index = plane
+    index unless planes are dropped)."""
+    keep = [k for k in range(48) if k not in drop]
+    return sparse_line_event([(1.0, 2.0, 0.01, 0.02, keep)], PLANES48)
+
+
+def test_full_record_overflow_hand_traced(oracle_mod):
+    """Seeds (47, 45, 43) at step 47 and (46, 44, 42) at step 46 (its h2 may
+    be used); both then take one plane per step (A.1).  A holds 24 hits at
+    step 26 (last hit 22), B at step 25 (last hit 21, taken at the step
+    where A, full, finds 21: A marks 21 used, Kernel.cl:316, and stops where
+    :321 would write past the record).  At step 24 B finds 20, marks it and
+    stops.  Hits 47 .. 20 are then used, so the next seed is (19, 17, 15)
+    at step 19 and (18, 16, 14) at step 18, which run to sensor 0.  Without
+    the mark of A's and B's 25th hits, hit 20 would stay free and seed
+    (20, 18, 16)."""
+    blob = _one_line_48()
+    out, st = oracle_mod.search_event(blob)
+    want = [[47, 45, 43] + list(range(42, 21, -1)),
+            [46, 44, 42] + list(range(41, 20, -1)),
+            [19, 17, 15] + list(range(14, -1, -1)),
+            [18, 16, 14] + list(range(13, -1, -1))]
+    assert _plane_tracks(blob, out) == want
+    assert st["n_full"] == 2 and st["n_overflow"] == 2 and st["flags"] == 1
+    sim, kern = simt.run_event(ev_mod.parse_event(blob))
+    assert sim == ev_mod.track_lists(out) and kern.overflows == 2
+
+
+def test_full_record_keeps_skipping(oracle_mod):
+    """As above with plane 21 empty: A, full since step 26, misses it at step
+    25 and is re-queued with one skip like any track (:349-356); at step 24
+    it finds 20, marks it and stops.  B (23 hits) misses 21 too, takes 20 at
+    step 24 (24 hits) and at step 23 finds 19, marks it and stops.  The
+    next seeds are (18, 16, 14) and (17, 15, 13)."""
+    blob = _one_line_48(drop=(21,))
+    out, st = oracle_mod.search_event(blob)
+    want = [[47, 45, 43] + list(range(42, 21, -1)),
+            [46, 44, 42] + list(range(41, 21, -1)) + [20],
+            [18, 16, 14] + list(range(13, -1, -1)),
+            [17, 15, 13] + list(range(12, -1, -1))]
+    assert _plane_tracks(blob, out) == want
+    assert st["n_full"] == 2 and st["n_overflow"] == 2
+    sim, kern = simt.run_event(ev_mod.parse_event(blob))
+    assert sim == ev_mod.track_lists(out) and kern.overflows == 2
 
 
 # ---------------------------------------------------------------------------
