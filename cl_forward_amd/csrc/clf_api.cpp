@@ -12,8 +12,27 @@
 
 using namespace clf;
 
+static_assert(sizeof(EventResult) == sizeof(clf_event_stats),
+              "the kernel's result record is the C ABI's clf_event_stats");
+
 namespace {
 thread_local std::string g_last_error;
+
+// The no-allocation scratch array of each device: its address is looked up
+// once per device, with that device current, when the first context on it
+// is created, and shared by every context on the device.
+constexpr int kMaxDevices = 64;
+struct ScratchSlot {
+    bool looked_up = false;
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    hipError_t err = hipSuccess;
+};
+std::mutex g_scratch_mu;
+ScratchSlot g_scratch[kMaxDevices];
+
+// Events per call of clf_search_noalloc (host tables sized at creation).
+constexpr int kNoAllocMaxEvents = 4096;
 }  // namespace
 
 namespace clf {
@@ -182,6 +201,14 @@ struct clf_context {
     std::mutex spare_mu;
     Resident spare;
     PinnedBuf spare_blobs, spare_results, spare_prefix, spare_packed;
+    // clf_search_noalloc: the device's scratch array and host tables for
+    // its event descriptors, sized at creation
+    void* na_scratch = nullptr;
+    size_t na_bytes = 0;
+    hipError_t na_err = hipSuccess;
+    std::vector<EventDesc> na_desc;
+    std::vector<int32_t> na_order;
+    std::vector<double> na_cost;
 };
 
 struct clf_batch {
@@ -286,6 +313,23 @@ clf_status clf_context_create(int device, clf_context** out) {
         return hip_fail(e, "hipGetDeviceProperties");
     }
     ctx->n_cu = prop.multiProcessorCount;
+    if (device < kMaxDevices) {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        ScratchSlot& s = g_scratch[device];
+        if (!s.looked_up) {
+            s.err = noalloc_scratch(&s.ptr, &s.bytes);
+            if (s.err != hipSuccess) s.ptr = nullptr;
+            s.looked_up = true;
+        }
+        ctx->na_scratch = s.ptr;
+        ctx->na_bytes = s.bytes;
+        ctx->na_err = s.err;
+    } else {
+        ctx->na_err = hipErrorInvalidDevice;
+    }
+    ctx->na_desc.resize(kNoAllocMaxEvents);
+    ctx->na_order.resize(kNoAllocMaxEvents);
+    ctx->na_cost.resize(kNoAllocMaxEvents);
     std::snprintf(ctx->name, sizeof(ctx->name), "%s", prop.name[0] ? prop.name : prop.gcnArchName);
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
@@ -670,6 +714,115 @@ clf_status clf_fill_candidates_search_path(clf_context* ctx, int32_t n,
                                            int32_t* const* cand_h1, int32_t* const* cand_h2,
                                            uint32_t* flags) {
     return fill_windows(ctx, n, events, sizes, cand_h1, cand_h2, flags, 2);
+}
+
+clf_status clf_search_noalloc(clf_context* ctx, clf_mode mode, int32_t n,
+                              const uint8_t* const* events, const size_t* sizes, uint8_t* tracks,
+                              int64_t track_capacity, int64_t* track_offset,
+                              clf_event_stats* stats) {
+    if (!ctx || n < 0 || (n > 0 && (!events || !sizes || !stats)) || !track_offset ||
+        track_capacity < 0 || (track_capacity > 0 && !tracks))
+        return fail(CLF_E_INVALID_ARGUMENT, "null argument or negative count");
+    if (mode != CLF_MODE_PLAIN && mode != CLF_MODE_BATCHED)
+        return fail(CLF_E_INVALID_ARGUMENT, "unknown mode");
+    if (!ctx->na_scratch) return hip_fail(ctx->na_err, "no-allocation scratch (hipGetSymbolAddress)");
+    if (n > kNoAllocMaxEvents)
+        return fail(CLF_E_CAPACITY, "clf_search_noalloc takes at most " +
+                                        std::to_string(kNoAllocMaxEvents) + " events per call, got " +
+                                        std::to_string(n));
+    HIP_TRY(current_device(ctx));
+    // event table, in the context's host arrays
+    EventDesc* desc = ctx->na_desc.data();
+    int32_t* order = ctx->na_order.data();
+    double* cost = ctx->na_cost.data();
+    size_t off = 0;
+    int64_t hits = 0, cap = 0;
+    for (int i = 0; i < n; ++i) {
+        int32_t S = 0, N = 0;
+        EventDesc& d = desc[i];
+        d.hit_off = hits;
+        d.track_off = cap;
+        cost[i] = 0.0;
+        if (validate_event(events[i], sizes[i], S, N)) {
+            d.blob_off = (int64_t)off;
+            d.n_hits = N;
+            d.n_sensors = S;
+            off += align_up(event_bytes(S, N), 256);
+            hits += N;
+            cap += CLF_EVENT_TRACK_CAPACITY(N, S);
+            cost[i] = event_cost(events[i], S, N);
+        } else {
+            d.blob_off = 0;
+            d.n_hits = -1;
+            d.n_sensors = 0;
+        }
+        order[i] = i;
+    }
+    // largest estimated cost first, ties by index: the order clf_batch_create
+    // gives (std::sort allocates nothing)
+    std::sort(order, order + n, [&](int x, int y) { return cost[x] != cost[y] ? cost[x] > cost[y] : x < y; });
+    // the call's buffers, carved out of the scratch array
+    size_t at = 0;
+    auto carve = [&](size_t bytes) {
+        const size_t p = at;
+        at = align_up(at + std::max<size_t>(bytes, 1), 256);
+        return p;
+    };
+    const size_t o_blobs = carve(off), o_desc = carve(sizeof(EventDesc) * (size_t)n),
+                 o_order = carve(sizeof(int32_t) * (size_t)n),
+                 o_queue = carve(sizeof(int32_t) * ((size_t)n + 1)),
+                 o_cand = carve(sizeof(int4) * (size_t)hits), o_used = carve((size_t)hits),
+                 o_tracks = carve(sizeof(clf_track) * (size_t)cap),
+                 o_ttf_a = carve(sizeof(TtfEntry) * (size_t)cap),
+                 o_ttf_b = carve(sizeof(TtfEntry) * (size_t)cap),
+                 o_weak = carve(sizeof(uint4) * (size_t)cap),
+                 o_results = carve(sizeof(EventResult) * (size_t)n),
+                 o_prefix = carve(sizeof(int64_t) * ((size_t)n + 1)),
+                 o_packed = carve(sizeof(clf_track) * (size_t)cap);
+    if (at > ctx->na_bytes)
+        return fail(CLF_E_CAPACITY, "these events need " + std::to_string(at) +
+                                        " bytes of device scratch; clf_search_noalloc has " +
+                                        std::to_string(ctx->na_bytes));
+    uint8_t* base = static_cast<uint8_t*>(ctx->na_scratch);
+    Resident r;  // a view: nothing here owns or frees these pointers
+    DevBuf* bufs[] = {&r.blobs, &r.desc,  &r.order, &r.queue,   &r.cand,   &r.used,  &r.tracks,
+                      &r.ttf_a, &r.ttf_b, &r.weak,  &r.results, &r.prefix, &r.packed};
+    const size_t offs[] = {o_blobs, o_desc,  o_order, o_queue,   o_cand,   o_used,  o_tracks,
+                           o_ttf_a, o_ttf_b, o_weak,  o_results, o_prefix, o_packed};
+    for (int k = 0; k < 13; ++k) bufs[k]->p = base + offs[k];
+    hipStream_t st = ctx->stream;
+    for (int i = 0; i < n; ++i) {
+        if (desc[i].n_hits < 0) continue;
+        HIP_TRY(hipMemcpyAsync(base + o_blobs + desc[i].blob_off, events[i],
+                               event_bytes(desc[i].n_sensors, desc[i].n_hits),
+                               hipMemcpyHostToDevice, st));
+    }
+    if (n > 0) {
+        HIP_TRY(hipMemcpyAsync(r.desc.p, desc, sizeof(EventDesc) * (size_t)n, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(r.order.p, order, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, st));
+    }
+    clf_status s = enqueue_search(r, n, mode, ctx->cfg, ctx->n_cu, st, nullptr, nullptr, ctx->side,
+                                  ctx->fork, ctx->join);
+    if (s != CLF_OK) return s;
+    if (n == 0) {
+        track_offset[0] = 0;
+        return CLF_OK;
+    }
+    HIP_TRY(hipMemcpyAsync(stats, r.results.p, sizeof(EventResult) * (size_t)n, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(track_offset, r.prefix.p, sizeof(int64_t) * ((size_t)n + 1),
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    const int64_t total = track_offset[n];
+    if (total < 0 || total > cap) return fail(CLF_E_DEVICE, "inconsistent track count");
+    if (total > track_capacity)
+        return fail(CLF_E_CAPACITY, "the events have " + std::to_string(total) +
+                                        " tracks; the output holds " + std::to_string(track_capacity));
+    if (total > 0) {
+        HIP_TRY(hipMemcpyAsync(tracks, r.packed.p, sizeof(clf_track) * (size_t)total,
+                               hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    return CLF_OK;
 }
 
 clf_status clf_search(clf_context* ctx, clf_mode mode, int32_t n, const uint8_t* const* events,

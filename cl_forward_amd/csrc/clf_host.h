@@ -23,6 +23,9 @@ hipError_t occupancy(int block, int lds_hits, int* blocks_per_cu);
 hipError_t launch_probe_copy(const void* src, void* dst, int64_t bytes, hipStream_t stream);
 hipError_t launch_serve(const ServeArgs& a, int block, int lds_hits, int grid, hipStream_t stream);
 hipError_t serve_occupancy(int block, int lds_hits, int* blocks_per_cu);
+// Device address and size of the no-allocation scratch array of the
+// current device (hipGetSymbolAddress; call once per device, with it current).
+hipError_t noalloc_scratch(void** ptr, size_t* bytes);
 
 // Error reporting: the message is kept per thread for clf_last_error().
 clf_status fail(clf_status st, const std::string& msg);

@@ -2016,6 +2016,21 @@ __global__ __launch_bounds__(256) void probe_copy_kernel(const uint4* __restrict
 }
 
 // ---------------------------------------------------------------------------
+// Device scratch of the entry point that allocates nothing
+// (clf_search_noalloc): every device buffer of one of its calls is carved
+// out of this array.  Each device that loads the library holds its own copy
+// until the process ends.  Host code never uses the array's name: the
+// address comes from noalloc_scratch(), once per device.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+hipError_t noalloc_scratch(void** ptr, size_t* bytes) {
+    *bytes = kNoAllocScratchBytes;
+    return hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch));
+}
+
+// ---------------------------------------------------------------------------
 // Launchers (called from the host library)
 // ---------------------------------------------------------------------------
 #define CLF_FOR_EACH_CONFIG(X) \

@@ -125,6 +125,26 @@ class Context:
             b.run(mode)
             return b.results()
 
+    def search_noalloc(self, events, mode: int = MODE_BATCHED, capacity: int | None = None):
+        """clf_search_noalloc: the search without any device allocation (the
+        library's static scratch array).  Returns (list of output blobs,
+        list of stats); capacity (output records) defaults to the events'
+        bound."""
+        n = len(events)
+        holders, ptrs, sizes = _event_arrays(events)
+        if capacity is None:
+            capacity = sum(max(track_capacity(e), 0) for e in events)
+        tracks = (ctypes.c_uint8 * max(capacity * TRACK_BYTES, 1))()
+        offs = (ctypes.c_int64 * (n + 1))()
+        stats = (EventStats * max(n, 1))()
+        _lib.check(self._L.clf_search_noalloc(self._h, int(mode), n, ptrs, sizes, _u8(tracks),
+                                              int(capacity), offs, stats),
+                   "clf_search_noalloc")
+        del holders
+        raw = bytes(tracks)
+        outs = [raw[offs[i] * TRACK_BYTES: offs[i + 1] * TRACK_BYTES] for i in range(n)]
+        return outs, [{k: int(getattr(stats[i], k)) for k, _ in EventStats._fields_} for i in range(n)]
+
     def fill_candidates(self, events, search_path: bool = False):
         """Stage (a) alone.  Returns a list of (cand_h1, cand_h2, flags) per event,
         the windows as int32 numpy arrays of 2*N.  search_path: computed as

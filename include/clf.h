@@ -141,6 +141,27 @@ clf_status clf_search(clf_context* ctx, clf_mode mode, int32_t n_events,
                       const uint8_t* const* events, const size_t* sizes, clf_result* out,
                       clf_batch** batch_out);
 
+/* No counterpart in the reference.  For callers that cannot allocate
+ * device memory (tight serving loops, harnesses that forbid hipMalloc):
+ * clf_search without any allocation.  Every device buffer of the call is
+ * carved out of one static device array of 128 MiB that the library holds
+ * on each device it runs on (its address is looked up once per device when
+ * the first context there is created); nothing is allocated or freed, on
+ * the device or in pinned memory.  At most 4096 events per call, and what
+ * they need (about 2 MB per event of 2,800 hits, 12 MB at 32,768 hits)
+ * must fit the array: otherwise CLF_E_CAPACITY with the sizes in
+ * clf_last_error(), before anything is enqueued.  The caller passes host
+ * inputs as clf_search takes them and host outputs: stats[n_events],
+ * track_offset[n_events + 1] (as clf_result's) and tracks (track_capacity
+ * records; too few gives CLF_E_CAPACITY after the search, with stats and
+ * track_offset filled).  Results are those of clf_search.  Calls share the
+ * device's array and leave their contents in it: calls that overlap on one
+ * device (from any context or thread) race and must not be made. */
+clf_status clf_search_noalloc(clf_context* ctx, clf_mode mode, int32_t n_events,
+                              const uint8_t* const* events, const size_t* sizes, uint8_t* tracks,
+                              int64_t track_capacity, int64_t* track_offset,
+                              clf_event_stats* stats);
+
 /* Stage (a) alone: the candidate windows of every hit, as fillCandidates
  * leaves dev_hit_candidates and dev_hit_h2_candidates (Kernel.cl:50-178,
  * -1 where it writes nothing).  cand_h1[i] and cand_h2[i] hold 2*N_i int32
