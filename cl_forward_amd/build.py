@@ -22,6 +22,7 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "libclf.so")
 STANDALONE = os.path.join(PKG, "clf_standalone")
+DROPIN = os.path.join(PKG, "clf_dropin")
 SOURCES = ["clf_kernels.hip", "clf_api.cpp", "clf_serve.cpp", "clf_results.cpp", "clf_presort.cpp",
            "clf_shims.cpp"]
 
@@ -76,16 +77,17 @@ def build(arch: str = "gfx950", jobs: int = 4, verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
-    # the standalone program, host code over the library
-    exe_tmp = STANDALONE + ".tmp"
-    cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", os.path.join(CSRC, "clf_standalone.cpp"),
-           f"-L{PKG}", "-lclf", "-Wl,-rpath,$ORIGIN", "-o", exe_tmp]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"standalone link failed\n{r.stdout}\n{r.stderr}")
-    os.replace(exe_tmp, STANDALONE)
+    # the standalone program and the drop-in caller, host code over the library
+    for exe, src in ((STANDALONE, "clf_standalone.cpp"), (DROPIN, "clf_dropin.cpp")):
+        exe_tmp = exe + ".tmp"
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", os.path.join(CSRC, src),
+               f"-L{PKG}", "-lclf", "-Wl,-rpath,$ORIGIN", "-o", exe_tmp]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"{src} link failed\n{r.stdout}\n{r.stderr}")
+        os.replace(exe_tmp, exe)
     return LIB
 
 

@@ -156,3 +156,23 @@ def unframe(raw: bytes) -> tuple[bytes, bytes]:
 
 def frame(blob: bytes, name: bytes = HANDLER_NAME) -> bytes:
     return struct.pack("<i", len(name)) + name + struct.pack("<i", len(blob)) + blob
+
+
+def write_frame_stream(path, blobs) -> None:
+    """Events as offload frames back to back (the input of clf_dropin)."""
+    with open(path, "wb") as f:
+        for b in blobs:
+            f.write(frame(bytes(b)))
+
+
+def read_sized_stream(path) -> list[bytes]:
+    """int64 size + bytes records back to back (the output of clf_dropin)."""
+    out = []
+    with open(path, "rb") as f:
+        data = f.read()
+    p = 0
+    while p < len(data):
+        (n,) = struct.unpack_from("<q", data, p)
+        out.append(data[p + 8 : p + 8 + n])
+        p += 8 + n
+    return out

@@ -97,6 +97,35 @@ def test_reference_caller_links_against_the_library(tmp_path):
     assert exe.exists()
 
 
+def test_dropin_caller_checks_its_input(tmp_path):
+    """clf_dropin reads the offload-frame stream before touching the device:
+    a missing file or a truncated frame is refused with status 2."""
+    exe = os.path.join(os.path.dirname(LIB), "clf_dropin")
+    assert os.path.exists(exe), "build clf_dropin first"
+    bad = tmp_path / "bad.frames"
+    good = ev_mod.frame(generator.single_event().pack())
+    bad.write_bytes(good + good[:-3])
+    for path in (str(tmp_path / "missing"), str(bad)):
+        r = subprocess.run([exe, path], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2 and "clf_dropin" in r.stderr
+
+
+def test_frame_streams_round_trip(tmp_path):
+    blobs = [generator.single_event().pack(), b"", generator.plain_event(2).pack()]
+    p = tmp_path / "s.frames"
+    ev_mod.write_frame_stream(p, blobs)
+    raw = p.read_bytes()
+    got, pos = [], 0
+    while pos < len(raw):
+        name, data = ev_mod.unframe(raw[pos:])
+        got.append(data)
+        pos += 8 + len(name) + len(data)
+    assert got == blobs
+    q = tmp_path / "s.sized"
+    q.write_bytes(b"".join(len(b).to_bytes(8, "little") + b for b in blobs))
+    assert ev_mod.read_sized_stream(q) == blobs
+
+
 def test_status_strings_and_no_device(lib):
     assert lib.clf_status_string(0) == b"ok"
     assert lib.clf_version().startswith(b"clf")
