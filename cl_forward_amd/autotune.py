@@ -3,10 +3,11 @@
 Sweeps the workgroup size {64, 128, 256, 512, 1024}, the hits per LDS sensor slot
 {256, 512} and the persistent grid (a multiple of the compute-unit count, or
 the occupancy-derived default) on a resident batch, then, for the best of
-those, the heavy-event lane (the costliest events in a second launch of
-512- or 1024-thread workgroups: 0, 8, 16, 32, 64 or 128 events); checks that every
-configuration returns the same bytes, and records the fastest in
-``cl_forward_amd/tuned.json``:
+those, the work-queue granularity (1, 2, 4, 8 or 16 consecutive events of the
+cost-ordered queue per dequeue) and the heavy-event lane (the costliest
+events in a second launch of 512- or 1024-thread workgroups: 0, 8, 16, 32, 64
+or 128 events); checks that every configuration returns the same bytes, and
+records the fastest in ``cl_forward_amd/tuned.json``:
 
     python -m cl_forward_amd.autotune [--workload batch1000] [--reps 5]
 
@@ -30,6 +31,10 @@ BLOCKS = (64, 128, 256, 512, 1024)
 LDS_HITS = (256, 512)
 GRID_PER_CU = (0, 1, 2, 4)  # 0 = occupancy-derived default
 HEAVY_EVENTS = (0, 8, 16, 32, 64, 128)
+GRAINS = (1, 2, 4, 8, 16)
+# a grain above 1 only if it wins by this much (it hands the costliest
+# events out in runs, which lengthens the batch's tail)
+GRAIN_MARGIN = 0.01
 HEAVY_BLOCKS = (512, 1024)
 # a second launch is kept only when it beats the single launch by this
 # fraction (run-to-run spread of the timing is a few per cent)
@@ -57,12 +62,17 @@ def pick_best(table: list[dict]) -> dict:
     """Fastest row; among single launches of the fastest shape, the smallest
     grid within GRID_MARGIN (0 = the resident workgroups); a heavy-event lane
     only if it wins by HEAVY_MARGIN."""
-    singles = [r for r in table if r["heavy_events"] == 0]
+    singles = [r for r in table if r["heavy_events"] == 0 and r.get("grain", 1) == 1]
     fastest = min(singles, key=lambda r: r["ms_run"])
     near = [r for r in singles
             if r["block_size"] == fastest["block_size"] and r["lds_hits"] == fastest["lds_hits"]
             and r["ms_run"] <= (1.0 + GRID_MARGIN) * fastest["ms_run"]]
     single = min(near, key=lambda r: (r["grid_size"], r["ms_run"]))
+    grained = [r for r in table if r["heavy_events"] == 0 and r.get("grain", 1) > 1]
+    if grained:
+        best_grain = min(grained, key=lambda r: r["ms_run"])
+        if best_grain["ms_run"] <= (1.0 - GRAIN_MARGIN) * single["ms_run"]:
+            single = best_grain
     heavy = [r for r in table if r["heavy_events"]]
     if heavy:
         best_heavy = min(heavy, key=lambda r: r["ms_run"])
@@ -94,14 +104,17 @@ def tune(ctx, batch, reps: int = 5, warmup: int = 1, log=print) -> dict:
 
     for block, lds_hits, per_cu in itertools.product(BLOCKS, LDS_HITS, GRID_PER_CU):
         measure(block_size=block, grid_size=per_cu * ctx.num_cus, lds_hits=lds_hits,
-                heavy_events=0, heavy_block=512)
+                grain=1, heavy_events=0, heavy_block=512)
     base = min(table, key=lambda r: r["ms_run"])
+    for g in GRAINS[1:]:
+        measure(block_size=base["block_size"], grid_size=base["grid_size"],
+                lds_hits=base["lds_hits"], grain=g, heavy_events=0, heavy_block=512)
     for hb, k in itertools.product(HEAVY_BLOCKS, HEAVY_EVENTS[1:]):
         measure(block_size=base["block_size"], grid_size=base["grid_size"],
-                lds_hits=base["lds_hits"], heavy_events=k, heavy_block=hb)
+                lds_hits=base["lds_hits"], grain=1, heavy_events=k, heavy_block=hb)
     best = pick_best(table)
     return {"best": {"block_size": best["block_size"], "grid_size": best["grid_size"],
-                     "lds_hits": best["lds_hits"], "grain": 1, "use_graph": 1,
+                     "lds_hits": best["lds_hits"], "grain": best.get("grain", 1), "use_graph": 1,
                      "heavy_events": best["heavy_events"], "heavy_block": best["heavy_block"]},
             "table": table}
 

@@ -122,6 +122,11 @@ SearchArgs Resident::args(int32_t n) const {
     a.ttf_b = ttf_b.as<TtfEntry>();
     a.weak = weak.as<uint4>();
     a.results = results.as<EventResult>();
+    // prefix[e]: first packed record of event e; prefix[n]: the allocator,
+    // which ends as the number of packed records
+    a.packed = packed.as<clf_track>();
+    a.track_base = prefix.as<int64_t>();
+    a.track_alloc = reinterpret_cast<unsigned long long*>(prefix.as<int64_t>() + n);
     a.fill_only = 0;
     a.prof = nullptr;
     return a;
@@ -142,9 +147,11 @@ clf_status enqueue_search(const Resident& r, int32_t n, clf_mode mode,
                           const clf_launch_config& cfg, int n_cu, hipStream_t st, hipEvent_t t0,
                           hipEvent_t t1, hipStream_t side, hipEvent_t fork, hipEvent_t join) {
     if (n == 0) return CLF_OK;
-    // queue counters: one per launch (n+1 so plain mode has one per event)
-    HIP_TRY(launch_zero_i32(r.queue.as<int32_t>(), n + 1, st));
     SearchArgs a = r.args(n);
+    a.grain = std::max(1, cfg.grain);
+    // queue counters, one per launch (n+1 so plain mode has one per event),
+    // and the packed-output allocator
+    HIP_TRY(launch_zero_i32(r.queue.as<int32_t>(), n + 1, st, a.track_alloc));
     if (t0) HIP_TRY(hipEventRecord(t0, st));
     const int heavy = side && fork && join && cfg.heavy_events > 0 ? std::min(cfg.heavy_events, n) : 0;
     if (mode == CLF_MODE_BATCHED && heavy > 0) {
@@ -179,9 +186,6 @@ clf_status enqueue_search(const Resident& r, int32_t n, clf_mode mode,
         }
     }
     if (t1) HIP_TRY(hipEventRecord(t1, st));
-    HIP_TRY(launch_pack(r.desc.as<EventDesc>(), r.results.as<EventResult>(), n,
-                        r.prefix.as<int64_t>(), r.tracks.as<clf_track>(),
-                        r.packed.as<uint32_t>(), st));
     return CLF_OK;
 }
 
@@ -374,7 +378,7 @@ clf_status clf_context_set_config(clf_context* ctx, const clf_launch_config* cfg
         return fail(CLF_E_INVALID_ARGUMENT,
                     "unsupported (block_size, lds_hits): block 64/128/256/512/1024, lds_hits 256/512");
     if (cfg->grid_size < 0) return fail(CLF_E_INVALID_ARGUMENT, "grid_size must be >= 0");
-    if (cfg->grain != 1) return fail(CLF_E_INVALID_ARGUMENT, "grain must be 1");
+    if (cfg->grain < 1 || cfg->grain > 64) return fail(CLF_E_INVALID_ARGUMENT, "grain must be 1..64");
     if (cfg->heavy_events < 0 ||
         (cfg->heavy_events > 0 && !config_supported(cfg->heavy_block, cfg->lds_hits)))
         return fail(CLF_E_INVALID_ARGUMENT, "heavy_events must be >= 0 with a supported heavy_block");
@@ -461,7 +465,7 @@ clf_status clf_batch_create(clf_context* ctx, int32_t n, const uint8_t* const* e
             d.n_sensors = S;
             off += align_up(8 + 12 * (size_t)S + 16 * (size_t)N, 256);
             hits += N;
-            cap += CLF_EVENT_TRACK_CAPACITY(N, S);
+            cap += track_slice(N, S);
             cost[i] = event_cost(events[i], S, N);
         } else {
             d.blob_off = 0;
@@ -532,6 +536,12 @@ clf_status clf_batch_results(clf_context* ctx, clf_batch* b, clf_result* out) {
         HIP_TRY(hipStreamSynchronize(st));
         const int64_t total = b->h_prefix.as<int64_t>()[n];
         if (total < 0 || total > b->total_cap) return fail(CLF_E_DEVICE, "inconsistent track count");
+        const clf_event_stats* stv = b->h_results.as<clf_event_stats>();
+        for (int i = 0; i < n; ++i) {
+            const int64_t a0 = b->h_prefix.as<int64_t>()[i];
+            if (a0 < 0 || stv[i].n_tracks < 0 || a0 + stv[i].n_tracks > total)
+                return fail(CLF_E_DEVICE, "inconsistent track offsets");
+        }
         HIP_TRY(b->h_packed.ensure(sizeof(clf_track) * (size_t)total));
         if (total > 0)
             HIP_TRY(hipMemcpyAsync(b->h_packed.p, b->dev.packed.p, sizeof(clf_track) * (size_t)total,
@@ -605,9 +615,9 @@ clf_status clf_batch_profile(clf_context* ctx, clf_batch* b, clf_mode mode, uint
         ~Guard() { p.release(); }
     } guard{prof};
     hipStream_t st = ctx->stream;
-    HIP_TRY(launch_zero_i32(b->dev.queue.as<int32_t>(), n + 1, st));
     SearchArgs a = b->dev.args(n);
     a.prof = prof.as<unsigned long long>();
+    HIP_TRY(launch_zero_i32(b->dev.queue.as<int32_t>(), n + 1, st, a.track_alloc));
     if (mode == CLF_MODE_BATCHED) {
         HIP_TRY(launch_search(a, ctx->cfg.block_size, ctx->cfg.lds_hits, resolved_grid(ctx->cfg, ctx->n_cu, n), st));
     } else {
@@ -749,7 +759,7 @@ clf_status clf_search_noalloc(clf_context* ctx, clf_mode mode, int32_t n,
             d.n_sensors = S;
             off += align_up(event_bytes(S, N), 256);
             hits += N;
-            cap += CLF_EVENT_TRACK_CAPACITY(N, S);
+            cap += track_slice(N, S);
             cost[i] = event_cost(events[i], S, N);
         } else {
             d.blob_off = 0;
@@ -814,6 +824,9 @@ clf_status clf_search_noalloc(clf_context* ctx, clf_mode mode, int32_t n,
     HIP_TRY(hipStreamSynchronize(st));
     const int64_t total = track_offset[n];
     if (total < 0 || total > cap) return fail(CLF_E_DEVICE, "inconsistent track count");
+    for (int i = 0; i < n; ++i)
+        if (track_offset[i] < 0 || stats[i].n_tracks < 0 || track_offset[i] + stats[i].n_tracks > total)
+            return fail(CLF_E_DEVICE, "inconsistent track offsets");
     if (total > track_capacity)
         return fail(CLF_E_CAPACITY, "the events have " + std::to_string(total) +
                                         " tracks; the output holds " + std::to_string(track_capacity));

@@ -60,7 +60,7 @@ struct SearchArgs {
     int32_t first_item;    // items [first_item, first_item + n_items) of order
     int32_t n_items;
     int32_t* queue_head;   // work-queue counter, zero before the launch
-    int32_t grain;         // events taken per dequeue
+    int32_t grain;         // consecutive queue items taken per dequeue (>= 1)
     int4* cand;            // per hit, 16 B: stage (a) output of the fill-only
                            // check; otherwise W1Rec[N] then W2Rec[N] of
                            // sensors too large for an LDS slot
@@ -70,6 +70,13 @@ struct SearchArgs {
     TtfEntry* ttf_b;
     uint4* weak;           // per event, track_off entries: tracklet hits
     EventResult* results;  // per event
+    // Packed output (nullptr: none, the caller copies from `tracks`): at the
+    // end of event e its workgroup reserves n_tracks records at
+    // *track_alloc (zero before the launch), records the first one in
+    // track_base[e] and copies the event's tracks there.
+    clf_track* packed;
+    int64_t* track_base;
+    unsigned long long* track_alloc;
     int32_t fill_only;     // stop after the candidate windows (stage (a) check)
     // Optional per-event phase profile (nullptr: off): kProfSlots counters per
     // event, see the PROF_* indices in clf_kernels.hip.

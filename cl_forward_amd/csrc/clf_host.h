@@ -16,9 +16,8 @@ namespace clf {
 // Kernel launchers (clf_kernels.hip).
 hipError_t launch_search(const SearchArgs& a, int block, int lds_hits, int grid, hipStream_t stream);
 bool config_supported(int block, int lds_hits);
-hipError_t launch_zero_i32(int32_t* p, int n, hipStream_t stream);
-hipError_t launch_pack(const EventDesc* ev, const EventResult* res, int n, int64_t* prefix,
-                       const clf_track* tracks, uint32_t* packed, hipStream_t stream);
+// Zeros p[0..n) and, if given, the 64-bit counter *alloc (one kernel).
+hipError_t launch_zero_i32(int32_t* p, int n, hipStream_t stream, unsigned long long* alloc = nullptr);
 hipError_t occupancy(int block, int lds_hits, int* blocks_per_cu);
 hipError_t launch_probe_copy(const void* src, void* dst, int64_t bytes, hipStream_t stream);
 hipError_t launch_serve(const ServeArgs& a, int block, int lds_hits, int grid, hipStream_t stream);
@@ -125,6 +124,13 @@ inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 // Bytes of a well-formed event blob with S sensors and N hits.
 inline size_t event_bytes(int32_t S, int32_t N) { return 8 + 12 * (size_t)S + 16 * (size_t)N; }
+
+// Records of one event's slice of the per-track scratch (tracks, ttf lists,
+// weak entries): the capacity bound rounded up to 4 records, so that every
+// slice of the 100-byte records starts 16-byte aligned.
+inline int64_t track_slice(int32_t N, int32_t S) {
+    return (CLF_EVENT_TRACK_CAPACITY(N, S) + 3) & ~(int64_t)3;
+}
 
 // Header checks of one blob; fills S and N.  Same rules as the oracle.
 bool validate_event(const uint8_t* b, size_t size, int32_t& S, int32_t& N);

@@ -221,6 +221,7 @@ struct SharedCore {
     uint32_t g_prev, g_shift, g_counter;
     int32_t g_nv, g_more;
     int32_t n_listed;  // entries of the list the next step forwards
+    int64_t pack_base;  // first packed record of the current event
     // the current batch of gathered h0 (indices relative to sensor f)
     int32_t v_h0[kMaxVisits];
     int32_t v_first[kMaxVisits];  // h1 window start, relative to sensor f-2
@@ -1416,6 +1417,7 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
             EventResult r = {};
             r.flags = CLF_FLAG_INVALID;
             *res = r;
+            if (a.track_base) a.track_base[e] = 0;
         }
         return;
     }
@@ -1673,6 +1675,38 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
         r.n_overflow = sh.n_overflow;
         *res = r;
     }
+    if (a.packed) {
+        // the event's tracks into the packed output, at records reserved
+        // with one atomic: written where they are read back, with no pass
+        // of their own over the batch.  What the copy needs is re-read here
+        // (LDS, the descriptor) rather than held across the sensor loop.
+        if (tid == 0) {
+            const unsigned long long b = atomicAdd(a.track_alloc, (unsigned long long)n_tracks);
+            a.track_base[e] = (int64_t)b;
+            sh.pack_base = (int64_t)b;
+            sh.item = n_tracks;
+        }
+        __syncthreads();  // also orders the weak filter's track stores before the reads
+        const int64_t base = uni(sh.pack_base);
+        const int nt = uni(sh.item);
+        const int64_t toff = uni(a.ev[e].track_off);
+        // the source slice starts 16-byte aligned (slices are multiples of 4
+        // records); a destination record keeps 4-byte alignment only: 16-byte
+        // loads, 4-byte stores, then the words past the last 16-byte one
+        const uint4* src = reinterpret_cast<const uint4*>(a.tracks + toff);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(a.packed + base);
+        const int n16 = (nt * (int)sizeof(clf_track)) / 16;
+        for (int k = tid; k < n16; k += B) {
+            const uint4 v = src[k];
+            dst[4 * k] = v.x;
+            dst[4 * k + 1] = v.y;
+            dst[4 * k + 2] = v.z;
+            dst[4 * k + 3] = v.w;
+        }
+        const int words = nt * (int)(sizeof(clf_track) / 4);
+        const uint32_t* src4 = reinterpret_cast<const uint32_t*>(a.tracks + toff);
+        for (int k = 4 * n16 + tid; k < words; k += B) dst[k] = src4[k];
+    }
     prof_mark(sh, pc, PROF_TAIL);
     if (pc.on && tid == 0) {
         sh.prof[PROF_TOTAL] = pc.t - t_start;
@@ -1696,16 +1730,21 @@ template <int B, int C>
 __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(MinWaves<B, C>::value)))
 void search_kernel(SearchArgs a) {
     __shared__ Shared<B, C> sh;
+    const int grain = uni(a.grain);
     for (;;) {
-        if (threadIdx.x == 0) sh.item = atomicAdd(a.queue_head, 1);
+        // `grain` consecutive items of the cost-ordered queue per dequeue
+        if (threadIdx.x == 0) sh.item = atomicAdd(a.queue_head, grain);
         __syncthreads();
-        const int item = uni(sh.item);
+        const int item0 = uni(sh.item);
         __syncthreads();
-        if (item >= a.n_items) break;  // every wave of the group leaves together
-        const int idx = a.first_item + item;
-        const int e = uni(a.order ? a.order[idx] : idx);
-        process_event<B, C>(a, e, sh);
-        __syncthreads();
+        if (item0 >= a.n_items) break;  // every wave of the group leaves together
+        const int end = min(item0 + grain, a.n_items);
+        for (int item = item0; item < end; item = uni(item + 1)) {
+            const int idx = a.first_item + item;
+            const int e = uni(a.order ? a.order[idx] : idx);
+            process_event<B, C>(a, e, sh);
+            __syncthreads();
+        }
     }
 }
 
@@ -1957,53 +1996,13 @@ void serve_kernel(ServeArgs a) {
     }
 }
 
-// Writes zeros: the queue counters of a launch (a kernel, not a memset, so
-// that the sequence can be captured into a graph and replayed).
-__global__ void zero_i32_kernel(int32_t* p, int n) {
+// Writes zeros: the queue counters of a launch, and its packed-output
+// allocator (a kernel, not a memset, so that the sequence can be captured
+// into a graph and replayed).
+__global__ void zero_i32_kernel(int32_t* p, int n, unsigned long long* alloc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
-}
-
-// Exclusive scan of n_tracks over events into track_prefix[0..n] (one block).
-__global__ __launch_bounds__(1024) void track_prefix_kernel(const EventResult* res, int n,
-                                                            int64_t* prefix) {
-    __shared__ int64_t wsum[1024 / kWave];
-    __shared__ int64_t carry;
-    const int tid = threadIdx.x, lane = lane_id(), wave = wave_id();
-    if (tid == 0) carry = 0;
-    __syncthreads();
-    for (int base = 0; base < n; base += 1024) {
-        const int i = base + tid;
-        int64_t x = i < n ? (int64_t)res[i].n_tracks : 0;
-        int64_t incl = x;
-#pragma unroll
-        for (int d = 1; d < kWave; d <<= 1) {
-            const int64_t y = __shfl_up(incl, d);
-            if (lane >= d) incl += y;
-        }
-        if (lane == kWave - 1) wsum[wave] = incl;
-        __syncthreads();
-        int64_t before = carry;
-        for (int w = 0; w < wave; ++w) before += wsum[w];
-        if (i < n) prefix[i] = before + incl - x;
-        __syncthreads();
-        if (tid == 1023) carry = before + incl;
-        __syncthreads();
-    }
-    if (tid == 0) prefix[n] = carry;
-}
-
-// Copies each event's tracks into one packed array (one block per event).
-__global__ __launch_bounds__(256) void pack_tracks_kernel(const EventDesc* ev,
-                                                          const EventResult* res,
-                                                          const int64_t* prefix,
-                                                          const clf_track* tracks,
-                                                          uint32_t* packed) {
-    const int e = blockIdx.x;
-    const int64_t n_words = (int64_t)res[e].n_tracks * (sizeof(clf_track) / 4);
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(tracks + ev[e].track_off);
-    uint32_t* dst = packed + prefix[e] * (int64_t)(sizeof(clf_track) / 4);
-    for (int64_t w = threadIdx.x; w < n_words; w += blockDim.x) dst[w] = src[w];
+    if (alloc && i == 0) *alloc = 0ull;
 }
 
 // Plain 16-byte-per-lane copy: the known-size reference kernel against
@@ -2095,18 +2094,10 @@ hipError_t occupancy(int block, int lds_hits, int* blocks_per_cu) {
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_zero_i32(int32_t* p, int n, hipStream_t stream) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(zero_i32_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, p, n);
-    return hipGetLastError();
-}
-
-hipError_t launch_pack(const EventDesc* ev, const EventResult* res, int n, int64_t* prefix,
-                       const clf_track* tracks, uint32_t* packed, hipStream_t stream) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(track_prefix_kernel, dim3(1), dim3(1024), 0, stream, res, n, prefix);
-    hipLaunchKernelGGL(pack_tracks_kernel, dim3(n), dim3(256), 0, stream, ev, res, prefix, tracks,
-                       packed);
+hipError_t launch_zero_i32(int32_t* p, int n, hipStream_t stream, unsigned long long* alloc) {
+    if (n <= 0 && !alloc) return hipSuccess;
+    const int m = n > 1 ? n : 1;
+    hipLaunchKernelGGL(zero_i32_kernel, dim3((m + 255) / 256), dim3(256), 0, stream, p, n, alloc);
     return hipGetLastError();
 }
 

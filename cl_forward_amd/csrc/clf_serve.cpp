@@ -421,7 +421,7 @@ clf_status poll_slot(clf_server* s, int k, bool& progress) {
                 r.out.status = CLF_OK;
                 r.out.batch_events = n;
                 r.out.stats = stats[i];
-                r.out.n_tracks = pre[i + 1] - pre[i];
+                r.out.n_tracks = stats[i].n_tracks;
                 r.out.tracks = packed + sizeof(clf_track) * (size_t)pre[i];
                 r.out.t_done_ns = t;
                 r.out_buf = sl.out_buf;
@@ -507,6 +507,9 @@ ServeArgs serve_args(clf_server* s) {
     b.ttf_b = s->wg_ttf_b.as<TtfEntry>();
     b.weak = s->wg_weak.as<uint4>();
     b.results = s->wg_results.as<EventResult>();
+    b.packed = nullptr;  // serve_one copies the tracks to the request itself
+    b.track_base = nullptr;
+    b.track_alloc = nullptr;
     b.fill_only = 0;
     b.prof = nullptr;
     void* p = nullptr;
@@ -713,7 +716,7 @@ clf_status create_resident(clf_server* s) {
     if (per_cu < 1) return fail(CLF_E_DEVICE, "serving kernel does not fit on a compute unit");
     s->grid = per_cu * s->n_cu;  // every workgroup resident at once; 0 is the feeder
     s->wg_hits = c.max_event_hits;
-    s->wg_cap = CLF_EVENT_TRACK_CAPACITY(s->wg_hits, CLF_MAX_SENSORS);
+    s->wg_cap = track_slice(s->wg_hits, CLF_MAX_SENSORS);
     s->wg_blob_bytes = (int64_t)align_up(event_bytes(CLF_MAX_SENSORS, (int32_t)s->wg_hits), kBlobAlign);
     const size_t g = (size_t)s->grid;
     int ring_n = 1, comp_n = 1;
@@ -816,7 +819,7 @@ clf_status submit_events(clf_server* s, int32_t n, const uint8_t* const* events,
         }
         e.bytes = event_bytes(e.S, e.N);
         e.span = align_up(e.bytes, kBlobAlign);
-        e.cap = CLF_EVENT_TRACK_CAPACITY(e.N, e.S);
+        e.cap = track_slice(e.N, e.S);
         e.out_cap = std::min<int64_t>(e.cap, (int64_t)s->conf.out_min_tracks +
                                                  (int64_t)e.N * s->conf.out_tracks_per_1k_hits / 1000);
         e.out_span = align_up(sizeof(clf_track) * (size_t)std::max<int64_t>(e.out_cap, 1), kBlobAlign);

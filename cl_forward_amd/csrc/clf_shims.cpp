@@ -106,7 +106,7 @@ int invoke_locked(int start, int count, const std::vector<const std::vector<uint
     TimingReport timing;
     if (g_verbosity >= 3 && count > 0) run_timing(ctx, batch, timing);
     for (int i = 0; i < count; ++i) {
-        const int64_t a = res.track_offset[i], b = res.track_offset[i + 1];
+        const int64_t a = res.track_offset[i], b = a + res.stats[i].n_tracks;
         std::vector<uint8_t>& out = output[start + i];
         out.resize((size_t)(b - a) * sizeof(clf_track));
         if (b > a) std::memcpy(out.data(), res.tracks + a * sizeof(clf_track), out.size());

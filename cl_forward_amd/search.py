@@ -142,8 +142,9 @@ class Context:
                    "clf_search_noalloc")
         del holders
         raw = bytes(tracks)
-        outs = [raw[offs[i] * TRACK_BYTES: offs[i + 1] * TRACK_BYTES] for i in range(n)]
-        return outs, [{k: int(getattr(stats[i], k)) for k, _ in EventStats._fields_} for i in range(n)]
+        st = [{k: int(getattr(stats[i], k)) for k, _ in EventStats._fields_} for i in range(n)]
+        outs = [raw[offs[i] * TRACK_BYTES: (offs[i] + st[i]["n_tracks"]) * TRACK_BYTES] for i in range(n)]
+        return outs, st
 
     def fill_candidates(self, events, search_path: bool = False):
         """Stage (a) alone.  Returns a list of (cand_h1, cand_h2, flags) per event,
@@ -211,11 +212,12 @@ class Batch:
         offs = [int(r.track_offset[i]) for i in range(n + 1)]
         total = offs[-1] if n else 0
         raw = ctypes.string_at(r.tracks, total * TRACK_BYTES) if total else b""
-        outs = [raw[offs[i] * TRACK_BYTES: offs[i + 1] * TRACK_BYTES] for i in range(n)]
         stats = []
         for i in range(n):
             s = r.stats[i]
             stats.append({k: int(getattr(s, k)) for k, _ in EventStats._fields_})
+        outs = [raw[offs[i] * TRACK_BYTES: (offs[i] + stats[i]["n_tracks"]) * TRACK_BYTES]
+                for i in range(n)]
         return outs, stats
 
     PROFILE_FIELDS = ("prologue", "stage", "forward", "gather", "seed_setup", "stage_wait",

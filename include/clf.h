@@ -65,7 +65,8 @@ typedef struct clf_event_stats {
 typedef struct clf_launch_config {
     int32_t block_size; /* 64, 128, 256, 512 or 1024 threads per event workgroup */
     int32_t grid_size;  /* persistent workgroups; 0 = CUs x occupancy        */
-    int32_t grain;      /* events per dequeue (1)                            */
+    int32_t grain;      /* consecutive queue items (events, in cost order)
+                           a workgroup takes per dequeue, 1..64              */
     int32_t use_graph;  /* batched mode: capture and replay a hipGraph       */
     int32_t lds_hits;   /* hits per LDS sensor slot, 256 or 512; a sensor
                            with more hits is read from HBM                   */
@@ -80,7 +81,13 @@ typedef struct clf_launch_config {
 typedef struct clf_result {
     int32_t n_events;
     const clf_event_stats* stats; /* [n_events]                               */
-    const int64_t* track_offset;  /* [n_events + 1] record offsets in tracks  */
+    const int64_t* track_offset;  /* [n_events + 1]: event i's tracks are the
+                                     stats[i].n_tracks records from record
+                                     track_offset[i] of `tracks` on; entry
+                                     n_events is the number of records.  In
+                                     batched mode events are packed in the
+                                     order they finish, which may change from
+                                     run to run; each event's records do not */
     const uint8_t* tracks;        /* packed 100-byte Track records            */
 } clf_result;
 

@@ -88,6 +88,21 @@ def test_batch1000_costliest_and_sample_other_launches(oracle_mod, batch1000, bl
         _check(ctx, oracle_mod, blobs, f"batch1000/{block}x{lds_hits}")
 
 
+@pytest.mark.parametrize("grain", [2, 5, 16])
+def test_batch1000_queue_grain(oracle_mod, batch1000, grain):
+    """Several consecutive events of the cost-ordered queue per dequeue
+    (the last run shorter than the grain when it does not divide the
+    count) give the same bytes."""
+    idx = costliest_and_stride(batch1000, n_top=16, n_stride=51)
+    blobs = [batch1000[i] for i in idx]
+    _say(f"batch1000: {len(blobs)} events, queue grain {grain}")
+    with Context(0) as ctx:
+        cfg = autotune.load_tuned(ctx.num_cus, "batch1000") or {}
+        cfg.update(use_graph=1, grain=grain)
+        ctx.set_config(**cfg)
+        _check(ctx, oracle_mod, blobs, f"batch1000/grain{grain}")
+
+
 def test_batch1000_sample_plain_mode(oracle_mod, batch1000):
     idx = costliest_and_stride(batch1000, n_top=4, n_stride=12)
     blobs = [batch1000[i] for i in idx]

@@ -358,3 +358,18 @@ def test_autotune_prefers_resident_grid_within_margin():
     assert pick_best([big, slow_resident]) is big
     # the tie-break stays within the fastest launch shape
     assert pick_best([big, other_shape]) is big
+
+
+def test_autotune_keeps_grain_one_within_margin():
+    from cl_forward_amd.autotune import GRAIN_MARGIN, pick_best
+
+    def row(ms, grain):
+        return {"block_size": 512, "grid_size": 0, "lds_hits": 256, "heavy_events": 0,
+                "heavy_block": 512, "grain": grain, "ms_run": ms}
+
+    one = row(1.0, 1)
+    near = row(1.0 * (1.0 - GRAIN_MARGIN / 2), 4)
+    far = row(1.0 * (1.0 - 2 * GRAIN_MARGIN), 8)
+    assert pick_best([one, near]) is one
+    assert pick_best([one, near, far]) is far
+    assert pick_best([one, row(1.2, 2)]) is one
