@@ -141,6 +141,41 @@ def serving_runs(ctx, a):
             "rates": rows}
 
 
+def dropin_cpp(blobs, outs, device_ms, ctx, in_bytes, out_bytes, reps=10):
+    """gpuPixelSearchByTriplet called from C++ on the workload (host vectors
+    in and out), timed by the caller's steady clock."""
+    import subprocess
+    import tempfile
+
+    from cl_forward_amd import event as ev_mod
+
+    exe = os.path.join(ROOT, "cl_forward_amd", "clf_dropin")
+    h2d, d2h = ctx.probe_link(max(in_bytes, out_bytes), reps=5)
+    floor_ms = (in_bytes / (h2d * 1e9) + out_bytes / (d2h * 1e9)) * 1e3
+    with tempfile.TemporaryDirectory() as td:
+        src, dst = os.path.join(td, "events.frames"), os.path.join(td, "tracks.out")
+        ev_mod.write_frame_stream(src, blobs)
+        r = subprocess.run([exe, src, "--entry", "gaudi", "--reps", str(reps), "--warmup", "3",
+                            "--out", dst], capture_output=True, text=True, timeout=300)
+        if r.returncode != 0:
+            log(f"clf_dropin failed: {r.stderr[-500:]}")
+            return None
+        t = json.loads(r.stdout.strip().splitlines()[-1])
+        got = ev_mod.read_sized_stream(dst)
+    ms = t["median_ms"]
+    res = {"ms_per_call": round(ms, 4), "min_ms": round(t["min_ms"], 4),
+           "events_per_s": round(len(blobs) / (ms / 1e3), 1), "tracks_identical": got == outs,
+           "link_gbps": {"h2d": round(h2d, 1), "d2h": round(d2h, 1)},
+           "link_floor_ms": round(floor_ms, 4),
+           "vs_device_plus_floor": round(ms / (device_ms + floor_ms), 3),
+           "what": "gpuPixelSearchByTriplet from C++ (clf_dropin): host vectors in, host vectors "
+                   "out, median of 10 calls after 3; floor = input bytes / H2D GB/s + track bytes "
+                   "/ D2H GB/s of pinned copies"}
+    log(f"end to end from C++: {ms:.3f} ms per call (link floor {floor_ms:.3f} ms, "
+        f"{res['vs_device_plus_floor']}x device + floor)")
+    return res
+
+
 def load_traffic(workload: str):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -314,6 +349,13 @@ def main(argv=None):
                "what": "clf_search: allocate, upload, one batched run, read back (host in, host out)"}
         log(f"end to end: {dt * 1e3:.3f} ms per call")
 
+    # the same call from C++, as the Gaudi offload handler makes it:
+    # gpuPixelSearchByTriplet with host vectors in and out (clf_dropin),
+    # against the host link's floor for moving the events in and the tracks out
+    e2e_cpp = None
+    if not a.no_plain:
+        e2e_cpp = dropin_cpp(blobs, outs, dev["ms_run"], ctx, in_bytes, TRACK_BYTES * n_tracks)
+
     # roofline of the search kernel: algorithmic bytes per launch
     # (input read once + output records written once) over its duration
     alg_bytes = in_bytes + TRACK_BYTES * n_tracks
@@ -380,6 +422,7 @@ def main(argv=None):
         "plain": plain,
         "batched_vs_plain": round(value / plain["events_per_s"], 2) if plain else None,
         "end_to_end": e2e,
+        "end_to_end_cpp": e2e_cpp,
         "oracle_identical_events": oracle_identical,
         "overflow_tracks": overflow["tracks"],
         "overflow": overflow,

@@ -182,6 +182,8 @@ def load():
         "clf_search_noalloc": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int32, u8pp, sz_p, u8p,
                                               ctypes.c_int64, ctypes.POINTER(ctypes.c_int64),
                                               ctypes.POINTER(EventStats)]),
+        "clf_probe_link": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.c_int32,
+                                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
         "clf_server_default_config": (None, [ctypes.POINTER(ServerConfig)]),
         "clf_server_create": (ctypes.c_int, [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]),
         "clf_server_destroy": (None, [vp]),
@@ -222,7 +224,7 @@ EXPORTED = (
     "clf_context_destroy", "clf_context_set_config", "clf_context_get_config",
     "clf_context_num_cus", "clf_context_synchronize", "clf_event_track_capacity", "clf_batch_create", "clf_batch_destroy",
     "clf_batch_run", "clf_batch_results", "clf_batch_time", "clf_search", "clf_search_noalloc", "clf_presort_event",
-    "clf_fill_candidates", "clf_fill_candidates_search_path", "clf_probe_copy", "clf_batch_profile", "clf_server_default_config",
+    "clf_fill_candidates", "clf_fill_candidates_search_path", "clf_probe_copy", "clf_probe_link", "clf_batch_profile", "clf_server_default_config",
     "clf_server_create", "clf_server_destroy", "clf_server_submit", "clf_server_submit_many", "clf_server_wait",
     "clf_server_release", "clf_server_replay", "clf_tracks_digest", "clf_server_submit_framed",
     "clf_unframe", "clf_format_tracks", "clf_context_device_name",

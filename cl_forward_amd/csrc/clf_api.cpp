@@ -1,9 +1,12 @@
 // clf_api.cpp — host side of the C ABI (include/clf.h): contexts, resident
 // batches, launches, graphs, read-back and the host presort.
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
+#include <thread>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -433,8 +436,69 @@ void clf_batch_destroy(clf_batch* b) {
     delete b;
 }
 
-clf_status clf_batch_create(clf_context* ctx, int32_t n, const uint8_t* const* events,
-                            const size_t* sizes, clf_batch** out) {
+}  // extern "C"
+
+namespace {
+
+// Copies the events into the batch's pinned staging and from there to the
+// device, on the context stream.  Large batches are gathered by several
+// host threads in chunks of about 2 MiB, and each chunk's upload is issued
+// as soon as the chunk is complete, so that the host copies and the DMA
+// transfer overlap.
+hipError_t upload_blobs(clf_context* ctx, clf_batch* b, const uint8_t* const* events) {
+    const int n = b->n;
+    uint8_t* hb = b->h_blobs.as<uint8_t>();
+    uint8_t* db = b->dev.blobs.as<uint8_t>();
+    hipStream_t st = ctx->stream;
+    auto gather = [&](int i) {
+        const EventDesc& d = b->h_desc[i];
+        if (d.n_hits >= 0) std::memcpy(hb + d.blob_off, events[i], event_bytes(d.n_sensors, d.n_hits));
+    };
+    const int threads = copy_threads(b->blob_bytes);
+    if (threads <= 1) {
+        for (int i = 0; i < n; ++i) gather(i);
+        return b->blob_bytes ? hipMemcpyAsync(db, hb, b->blob_bytes, hipMemcpyHostToDevice, st)
+                             : hipSuccess;
+    }
+    // chunk k: events [first[k], first[k + 1]), staging bytes [lo[k], lo[k + 1])
+    std::vector<int> first;
+    std::vector<size_t> lo;
+    const size_t kChunk = size_t(2) << 20;
+    for (int i = 0; i < n; ++i) {
+        const size_t at = (size_t)b->h_desc[i].blob_off;
+        if (b->h_desc[i].n_hits < 0) continue;
+        if (first.empty() || at - lo.back() >= kChunk) {
+            first.push_back(i);
+            lo.push_back(at);
+        }
+    }
+    const int nc = (int)first.size();
+    first.push_back(n);
+    lo.push_back(b->blob_bytes);
+    std::unique_ptr<std::atomic<int>[]> done(new std::atomic<int>[nc]);
+    for (int k = 0; k < nc; ++k) done[k].store(0);
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        for (int k; (k = next.fetch_add(1)) < nc;) {
+            for (int i = first[k]; i < first[k + 1]; ++i) gather(i);
+            done[k].store(1, std::memory_order_release);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t) pool.emplace_back(work);
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < nc; ++k) {
+        while (!done[k].load(std::memory_order_acquire)) std::this_thread::yield();
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(db + lo[k], hb + lo[k], lo[k + 1] - lo[k], hipMemcpyHostToDevice, st);
+    }
+    for (std::thread& t : pool) t.join();
+    return e;
+}
+
+// clf_batch_create; without `sync` the uploads are only enqueued.
+clf_status batch_create(clf_context* ctx, int32_t n, const uint8_t* const* events,
+                        const size_t* sizes, clf_batch** out, bool sync) {
     if (!ctx || !out || n < 0 || (n > 0 && (!events || !sizes)))
         return fail(CLF_E_INVALID_ARGUMENT, "null argument or negative count");
     *out = nullptr;
@@ -495,25 +559,28 @@ clf_status clf_batch_create(clf_context* ctx, int32_t n, const uint8_t* const* e
         clf_batch_destroy(b);
         return hip_fail(e, "batch allocation");
     }
-    uint8_t* hb = b->h_blobs.as<uint8_t>();
-    for (int i = 0; i < n; ++i) {
-        const EventDesc& d = b->h_desc[i];
-        if (d.n_hits < 0) continue;
-        std::memcpy(hb + d.blob_off, events[i], 8 + 12 * (size_t)d.n_sensors + 16 * (size_t)d.n_hits);
-    }
     hipStream_t st = ctx->stream;
-    chk(hipMemcpyAsync(b->dev.blobs.p, hb, off, hipMemcpyHostToDevice, st));
+    chk(upload_blobs(ctx, b, events));
     chk(hipMemcpyAsync(b->dev.desc.p, b->h_desc.data(), sizeof(EventDesc) * (size_t)n,
                        hipMemcpyHostToDevice, st));
     chk(hipMemcpyAsync(b->dev.order.p, b->h_order.data(), sizeof(int32_t) * (size_t)n,
                        hipMemcpyHostToDevice, st));
-    chk(hipStreamSynchronize(st));
+    if (sync) chk(hipStreamSynchronize(st));
     if (e != hipSuccess) {
         clf_batch_destroy(b);
         return hip_fail(e, "batch upload");
     }
     *out = b;
     return CLF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+clf_status clf_batch_create(clf_context* ctx, int32_t n, const uint8_t* const* events,
+                            const size_t* sizes, clf_batch** out) {
+    return batch_create(ctx, n, events, sizes, out, true);
 }
 
 clf_status clf_batch_run(clf_context* ctx, clf_batch* b, clf_mode mode) {
@@ -713,6 +780,43 @@ clf_status fill_windows(clf_context* ctx, int32_t n, const uint8_t* const* event
 
 extern "C" {
 
+clf_status clf_probe_link(clf_context* ctx, int64_t bytes, int32_t reps, double* h2d_gbps,
+                          double* d2h_gbps) {
+    if (!ctx || bytes < 1 || reps < 1) return fail(CLF_E_INVALID_ARGUMENT, "bad argument");
+    HIP_TRY(current_device(ctx));
+    DevBuf dev;
+    PinnedBuf host;
+    struct Guard {
+        DevBuf& d;
+        PinnedBuf& h;
+        ~Guard() {
+            d.release();
+            h.release();
+        }
+    } guard{dev, host};
+    HIP_TRY(dev.ensure((size_t)bytes));
+    HIP_TRY(host.ensure((size_t)bytes));
+    std::memset(host.p, 1, (size_t)bytes);
+    double gbps[2] = {0.0, 0.0};
+    for (int dir = 0; dir < 2; ++dir) {
+        auto copy = [&]() {
+            return dir == 0 ? hipMemcpyAsync(dev.p, host.p, (size_t)bytes, hipMemcpyHostToDevice, ctx->stream)
+                            : hipMemcpyAsync(host.p, dev.p, (size_t)bytes, hipMemcpyDeviceToHost, ctx->stream);
+        };
+        HIP_TRY(copy());  // warm-up
+        HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+        for (int i = 0; i < reps; ++i) HIP_TRY(copy());
+        HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+        HIP_TRY(hipEventSynchronize(ctx->ev1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        gbps[dir] = ms > 0.f ? (double)bytes * reps / (ms * 1e-3) / 1e9 : 0.0;
+    }
+    if (h2d_gbps) *h2d_gbps = gbps[0];
+    if (d2h_gbps) *d2h_gbps = gbps[1];
+    return CLF_OK;
+}
+
 clf_status clf_fill_candidates(clf_context* ctx, int32_t n, const uint8_t* const* events,
                                const size_t* sizes, int32_t* const* cand_h1,
                                int32_t* const* cand_h2, uint32_t* flags) {
@@ -842,10 +946,14 @@ clf_status clf_search(clf_context* ctx, clf_mode mode, int32_t n, const uint8_t*
                       const size_t* sizes, clf_result* out, clf_batch** batch_out) {
     if (!ctx || !out || !batch_out) return fail(CLF_E_INVALID_ARGUMENT, "null argument");
     *batch_out = nullptr;
+    if (mode != CLF_MODE_PLAIN && mode != CLF_MODE_BATCHED)
+        return fail(CLF_E_INVALID_ARGUMENT, "unknown mode");
+    // upload, search and read-back in stream order with no host wait in
+    // between, and no graph for a batch that runs once
     clf_batch* b = nullptr;
-    clf_status s = clf_batch_create(ctx, n, events, sizes, &b);
+    clf_status s = batch_create(ctx, n, events, sizes, &b, false);
     if (s != CLF_OK) return s;
-    s = clf_batch_run(ctx, b, mode);
+    s = enqueue_run(ctx, b, mode, false);
     if (s == CLF_OK) s = clf_batch_results(ctx, b, out);
     if (s != CLF_OK) {
         clf_batch_destroy(b);

@@ -5,8 +5,12 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "clf.h"
 #include "clf_device.h"
@@ -113,6 +117,28 @@ struct PinnedBuf {
         }
     }
 };
+
+// Runs fn(k) for every k in [0, n) on `threads` host threads (the caller
+// among them), each taking the next k from a shared counter.
+template <class F>
+void parallel_for(int n, int threads, F&& fn) {
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        for (int k; (k = next.fetch_add(1, std::memory_order_relaxed)) < n;) fn(k);
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < std::min(threads, n); ++t) pool.emplace_back(work);
+    work();
+    for (std::thread& t : pool) t.join();
+}
+
+// Host threads for copies of `bytes` bytes: one below 4 MiB, else up to 8
+// (no more than the machine has).
+inline int copy_threads(size_t bytes) {
+    if (bytes < (size_t(4) << 20)) return 1;
+    const unsigned hw = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(8u, hw ? hw : 1u));
+}
 
 inline int32_t rd_i32(const uint8_t* p) {
     int32_t v;

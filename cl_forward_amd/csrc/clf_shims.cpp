@@ -1,5 +1,7 @@
 // clf_shims.cpp — the reference's C++ entry points (GpuPixelSearchByTriplet.h,
 // KernelInvoker.h) as thin shims over the C ABI of include/clf.h.
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -7,6 +9,7 @@
 #include <iomanip>
 #include <iostream>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "GpuPixelSearchByTriplet.h"
@@ -105,11 +108,28 @@ int invoke_locked(int start, int count, const std::vector<const std::vector<uint
     if (s != CLF_OK) fatal(s, "clf_search");
     TimingReport timing;
     if (g_verbosity >= 3 && count > 0) run_timing(ctx, batch, timing);
-    for (int i = 0; i < count; ++i) {
-        const int64_t a = res.track_offset[i], b = a + res.stats[i].n_tracks;
-        std::vector<uint8_t>& out = output[start + i];
-        out.resize((size_t)(b - a) * sizeof(clf_track));
-        if (b > a) std::memcpy(out.data(), res.tracks + a * sizeof(clf_track), out.size());
+    // each event's records into its output vector (KernelInvoker.cpp:197-206),
+    // on several host threads when there is much to copy
+    const int64_t total = count > 0 ? res.track_offset[count] : 0;
+    const int threads = total * (int64_t)sizeof(clf_track) >= (int64_t(4) << 20) ? 8 : 1;
+    const int per = (count + 4 * threads - 1) / (4 * threads);
+    auto scatter = [&](int k) {
+        for (int i = k * per; i < std::min(count, (k + 1) * per); ++i) {
+            const uint8_t* src = res.tracks + res.track_offset[i] * sizeof(clf_track);
+            output[start + i].assign(src, src + (size_t)res.stats[i].n_tracks * sizeof(clf_track));
+        }
+    };
+    const int chunks = per > 0 ? (count + per - 1) / per : 0;
+    if (threads <= 1) {
+        for (int k = 0; k < chunks; ++k) scatter(k);
+    } else {
+        std::atomic<int> next{0};
+        std::vector<std::thread> pool;
+        for (int t = 0; t < threads; ++t)
+            pool.emplace_back([&]() {
+                for (int k; (k = next.fetch_add(1)) < chunks;) scatter(k);
+            });
+        for (std::thread& t : pool) t.join();
     }
     clf_batch_destroy(batch);
     if (g_verbosity >= 3 && count > 0) print_timing(timing);

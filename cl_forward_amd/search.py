@@ -116,6 +116,13 @@ class Context:
                    "clf_probe_copy")
         return float(ms.value)
 
+    def probe_link(self, nbytes: int, reps: int = 5):
+        """(host-to-device GB/s, device-to-host GB/s) of pinned copies of nbytes."""
+        h, d = ctypes.c_double(), ctypes.c_double()
+        _lib.check(self._L.clf_probe_link(self._h, int(nbytes), int(reps), ctypes.byref(h),
+                                          ctypes.byref(d)), "clf_probe_link")
+        return float(h.value), float(d.value)
+
     def batch(self, events) -> "Batch":
         return Batch(self, events)
 

@@ -203,6 +203,14 @@ clf_status clf_batch_profile(clf_context* ctx, clf_batch* batch, clf_mode mode, 
  * (bytes read + bytes written) calibrates HBM counters. */
 clf_status clf_probe_copy(clf_context* ctx, int64_t bytes, int32_t reps, float* ms_per_copy);
 
+/* Measurement probe, no counterpart in the reference: the host link's
+ * throughput for `bytes` between pinned host memory and the device, `reps`
+ * copies each way on the context stream; GB/s (1e9 bytes per second) of
+ * host-to-device and device-to-host copies.  The floor of a call that
+ * takes its events from the host and returns tracks to it. */
+clf_status clf_probe_link(clf_context* ctx, int64_t bytes, int32_t reps, double* h2d_gbps,
+                          double* d2h_gbps);
+
 /* ---- Serving ----------------------------------------------------------
  * A server accepts events one at a time and returns each event's tracks
  * as soon as they are ready.  The reference has no serving loop: its Gaudi
