@@ -125,7 +125,10 @@ def serving_runs(ctx, a):
                    "completed": summ["completed"], "rejected": summ["rejected"],
                    "identical_to_plain": same,
                    "mean_batch_events": round(summ["mean_batch_events"], 1),
+                   # arrival to dispatch: batches, the batch launch; resident,
+                   # the descriptor's publication (the event in device memory)
                    "mean_wait_ms": round(float(np.mean(rec["wait_ns"][ok])) / 1e6, 4) if ok.any() else None,
+                   "kernel_launches" if a.serve_engine == "resident" else "batches": summ["batches"],
                    "max_submit_lag_us": round(summ["max_submit_lag_ns"] / 1e3, 1),
                    "dev_us": {k: round(summ[f"dev_{k}_us"], 1) for k in ("copy_in", "search", "copy_out")},
                    "dev_busy_frac": round(summ["dev_busy_frac"], 3),

@@ -66,8 +66,6 @@ class ServerConfig(ctypes.Structure):
         ("max_batch_events", ctypes.c_int32),
         ("max_batch_hits", ctypes.c_int64),
         ("max_wait_us", ctypes.c_int32),
-        ("out_tracks_per_1k_hits", ctypes.c_int32),
-        ("out_min_tracks", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
     ]
 
@@ -88,6 +86,7 @@ class Served(ctypes.Structure):
         ("t_submit_ns", ctypes.c_int64),
         ("t_launch_ns", ctypes.c_int64),
         ("t_done_ns", ctypes.c_int64),
+        ("t_complete_ns", ctypes.c_int64),
     ]
 
 
@@ -100,6 +99,7 @@ class ReplayRecord(ctypes.Structure):
         ("status", ctypes.c_int32),
         ("batch_events", ctypes.c_int32),
         ("flags", ctypes.c_uint32),
+        ("device_ns", ctypes.c_int64),
     ]
 
 

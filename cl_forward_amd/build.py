@@ -70,7 +70,8 @@ def build(arch: str = "gfx950", jobs: int = 4, verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as pool:
         objs = list(pool.map(compile_one, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [cc, f"--offload-arch={arch}", "-shared", "-fPIC", "-pthread", *objs, "-o", tmp]
+    cmd = [cc, f"--offload-arch={arch}", "-shared", "-fPIC", "-pthread", *objs, "-o", tmp,
+           "-L/opt/rocm/lib", "-lhsa-runtime64", "-Wl,-rpath,/opt/rocm/lib"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -86,23 +86,26 @@ struct SearchArgs {
 constexpr int kProfSlots = 16;
 
 // ---- Resident serving kernel --------------------------------------------
-// A request published by the host in pinned memory.  `seq` is written last
-// (release): descriptor d of the ring is valid once seq == d + 1.
+// A request published by the host in pinned memory once its event blob is
+// in the device input arena.  `seq` is written last (release): descriptor d
+// of the ring is valid once seq == d + 1.
 struct alignas(16) ServeDesc {
-    uint64_t in_off;   // event blob in the input arena (256-byte aligned)
-    uint64_t out_off;  // byte offset of the request's output region (256-aligned)
+    uint64_t in_off;   // event blob in the device input arena (256-byte aligned)
+    uint64_t reserved;
     int32_t n_hits;
     int32_t n_sensors;
     int32_t req;       // host request index, echoed in the completion
-    int32_t out_cap;   // Track records the output region holds
+    int32_t out_cap;   // Track records the request may produce (its capacity bound)
     uint32_t seq;
     uint32_t pad[3];
 };
 static_assert(sizeof(ServeDesc) == 48, "the kernel reads a descriptor as 16-byte words");
 
 // A finished request, appended by the device to a ring in pinned memory;
-// valid once seq == position + 1.
-enum CompStatus { COMP_OK = 0, COMP_SPILL = 1 };  // SPILL: more tracks than out_cap, none written
+// valid once seq == position + 1.  Its tracks are the n_tracks records at
+// ring byte position out_pos (modulo the output ring's size) of the device
+// output ring.
+enum CompStatus { COMP_OK = 0, COMP_INVALID = 1 };  // INVALID: a descriptor outside the limits
 
 struct CompRec {
     uint32_t seq;
@@ -110,14 +113,16 @@ struct CompRec {
     int32_t wg;        // workgroup that ran it
     int32_t status;    // CompStatus
     EventResult res;
+    uint64_t out_pos;  // absolute byte position of its tracks in the output ring
     // device wall clock: claimed, event copied in, searched, tracks written
     uint64_t t_claim, t_copied, t_searched, t_written;
 };
 
 // Control words in device memory.  HEAD (descriptors handed to workers),
-// TAIL (descriptors seen published) and DONE (completions written) persist
-// across launches; QUIT, IDLE_TAIL and the feeder's heartbeat BEAT (64-bit,
-// words 6-7) are cleared before each launch.
+// TAIL (descriptors seen published), DONE (completions written) and the
+// output ring's allocator OUT_HEAD (64-bit, words 8-9) persist across
+// launches; QUIT, IDLE_TAIL and the feeder's heartbeat BEAT (64-bit, words
+// 6-7) are cleared before each launch.
 enum ServeCtl {
     CTL_HEAD = 0,
     CTL_TAIL = 1,
@@ -125,15 +130,16 @@ enum ServeCtl {
     CTL_DONE = 3,
     CTL_IDLE_TAIL = 4,
     CTL_BEAT = 6,
-    CTL_WORDS = 8
+    CTL_OUT_HEAD = 8,
+    CTL_WORDS = 16
 };
 
 struct ServeArgs {
     SearchArgs base;           // per-workgroup scratch: ev[w], results[w], slices by ev[w]
     const ServeDesc* ring;     // pinned, ring_mask + 1 entries
     uint32_t ring_mask;
-    const uint8_t* in_arena;   // pinned, in_bytes
-    clf_track* out_arena;      // pinned, out_bytes
+    const uint8_t* in_arena;   // device (fine-grained), in_bytes: the events
+    uint8_t* out_arena;        // device (fine-grained), out_bytes: the output ring
     uint64_t in_bytes, out_bytes;
     CompRec* comp;             // pinned, comp_mask + 1 entries
     uint32_t comp_mask;

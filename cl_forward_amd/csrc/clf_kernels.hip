@@ -231,7 +231,7 @@ struct SharedCore {
     unsigned long long v_best[kMaxVisits];
     unsigned long long prof[kProfSlots];
     // resident serving: the claimed descriptor
-    uint64_t sv_in, sv_out;
+    uint64_t sv_in;
     int32_t sv_n, sv_s, sv_req, sv_cap;
     uint32_t sv_d;
 };
@@ -1752,21 +1752,28 @@ void search_kernel(SearchArgs a) {
 // Resident serving kernel
 // ---------------------------------------------------------------------------
 // Workgroup 0 runs one wave, the feeder; every other workgroup is a worker.
-// A worker announces that it is idle by appending its index to the idle
-// ring (one fetch-add, never retried) and then watches its own mailbox.  The
-// feeder follows the host's descriptor ring in pinned memory and pairs
-// published descriptors with idle workers in order, up to 64 per round trip,
-// writing each descriptor number into the worker's mailbox.  A worker reads
-// its descriptor, copies the event from the pinned input arena into its own
-// scratch, runs process_event, copies the tracks into the request's pinned
-// output region and appends a completion record.
+// A worker announces that it is idle by appending an entry to the idle ring
+// (one fetch-add, never retried) and then watches its own mailbox.  The
+// feeder follows the host's descriptor ring in pinned memory (a descriptor
+// is published once its event is in the device input arena) and pairs
+// published descriptors with idle workers in order, up to 64 per round trip:
+// it claims each worker's idle entry with a compare-and-swap and writes the
+// descriptor number into the claimed workers' mailboxes.  A worker reads its
+// descriptor, copies the event from the device input arena into its own
+// scratch, runs process_event, reserves room for its tracks in the device
+// output ring (one atomic on the ring's allocator), writes them there and
+// appends a completion record to a ring in pinned memory; the host copies
+// the finished part of the output ring back with the copy engine.
 //
 // Termination: the feeder leaves when the host sets its stop word, or when
 // every published descriptor has been handed out and nothing has moved for
 // idle_ticks; it then sets CTL_QUIT.  A waiting worker leaves on CTL_QUIT
-// (after one more look at its mailbox), or when the feeder's heartbeat has
-// been silent for 4 x idle_ticks (the feeder is not running, so nothing can
-// be handed to it).  Every wait is bounded once the host stops publishing.
+// (after one more look at its mailbox: the feeder writes every hand-out
+// before CTL_QUIT), or when the feeder's heartbeat has been silent for
+// 4 x idle_ticks: then it withdraws its idle entry with a compare-and-swap
+// first, and if the feeder has claimed the entry already (the swap fails),
+// it stays for the hand-out, which the feeder writes right after the claim.
+// Every wait is bounded once the host stops publishing.
 
 // Polls are relaxed (no cache invalidation per poll); the one acquire per
 // event is the worker's check of its descriptor's sequence word.
@@ -1779,6 +1786,12 @@ __device__ __forceinline__ uint32_t poll_sys(const uint32_t* p) {
 // index of the first clear bit of m (64 if none)
 __device__ __forceinline__ int first_clear(uint64_t m) {
     return ~m == 0ull ? kWave : __builtin_ctzll(~m);
+}
+// idle-ring entry of position pos: (pos + 1) << 32 | worker, or, once the
+// feeder has claimed it or the worker has withdrawn it, one of these
+constexpr uint32_t kIdleClaimed = 0xfffffffeu, kIdleGone = 0xffffffffu;
+__device__ __forceinline__ uint64_t idle_entry(uint32_t pos, uint32_t who) {
+    return ((uint64_t)(pos + 1u) << 32) | (uint64_t)who;
 }
 
 // The feeder wave.  While descriptors keep coming it checks the next 64
@@ -1809,17 +1822,29 @@ __device__ void serve_feeder(const ServeArgs& a) {
             }
         }
         if (t != h) {
-            const uint64_t e = __hip_atomic_load(&a.idle_ring[(ih + (uint32_t)lane) & a.idle_mask],
-                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const bool idle = (uint32_t)(e >> 32) == ih + (uint32_t)lane + 1u;
-            const int ni = first_clear(ballot(idle));
-            const int na = min(ni, (int)(t - h));
-            if (lane < na)
-                __hip_atomic_store(&a.mailbox[(uint32_t)e], h + (uint32_t)lane + 1u,
+            // the written prefix of the idle ring from ih, at most one entry
+            // per waiting descriptor; each lane claims its entry, and the
+            // claimed ones take the descriptors in lane order (an entry its
+            // worker has withdrawn is passed over)
+            uint64_t* slot = &a.idle_ring[(ih + (uint32_t)lane) & a.idle_mask];
+            const uint64_t e = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t pos = ih + (uint32_t)lane;
+            const bool written = (uint32_t)(e >> 32) == pos + 1u;
+            const int nw = min(first_clear(ballot(written)), (int)(t - h));
+            bool claimed = false;
+            if (lane < nw && (uint32_t)e < kIdleClaimed) {
+                uint64_t expect = e;
+                claimed = __hip_atomic_compare_exchange_strong(
+                    slot, &expect, idle_entry(pos, kIdleClaimed), __ATOMIC_RELAXED,
+                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            const uint64_t cm = ballot(claimed);
+            if (claimed)
+                __hip_atomic_store(&a.mailbox[(uint32_t)e], h + (uint32_t)mbcnt(cm) + 1u,
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (na > 0) {
-                h += (uint32_t)na;
-                ih += (uint32_t)na;
+            if (nw > 0) {
+                h += (uint32_t)__popcll(cm);
+                ih += (uint32_t)nw;
                 moved = true;
             }
         }
@@ -1841,7 +1866,7 @@ __device__ void serve_feeder(const ServeArgs& a) {
 template <int B, int C>
 __device__ void serve_one(const ServeArgs& a, int w, Shared<B, C>& sh, uint64_t t_claim) {
     const int tid = threadIdx.x;
-    const uint64_t in_off = sh.sv_in, out_off = sh.sv_out;
+    const uint64_t in_off = sh.sv_in;
     const int N = sh.sv_n, S = sh.sv_s, req = sh.sv_req, out_cap = sh.sv_cap;
     const int64_t bytes = 8 + 12 * (int64_t)S + 16 * (int64_t)N;
     // the host checked these at submit; a descriptor outside them is
@@ -1849,16 +1874,20 @@ __device__ void serve_one(const ServeArgs& a, int w, Shared<B, C>& sh, uint64_t 
     const int64_t cap = CLF_EVENT_TRACK_CAPACITY(N, S);
     const bool fits = N >= 0 && S >= 0 && S <= CLF_MAX_SENSORS && N <= a.wg_hits &&
                       bytes <= a.wg_blob_bytes && cap <= a.wg_cap && out_cap >= 0 &&
-                      in_off + (uint64_t)bytes <= a.in_bytes &&
-                      out_off + (uint64_t)out_cap * sizeof(clf_track) <= a.out_bytes;
+                      (int64_t)out_cap * (int64_t)sizeof(clf_track) <= (int64_t)a.out_bytes &&
+                      in_off + (uint64_t)bytes <= a.in_bytes && (in_off & 15u) == 0;
     // the workgroup's own scratch (SearchArgs holds it as read-only input)
     uint8_t* blob = const_cast<uint8_t*>(a.base.blobs) + (size_t)w * (size_t)a.wg_blob_bytes;
     if (fits) {
-        // copy-in from pinned host memory, four 16-byte loads in flight per
-        // lane.  The search never reads the hit IDs (N words after the
-        // sensor table), so only the 16-byte words of the header and sensor
-        // table [0, h16) and those from the one holding X[0] on, [x16, n16),
-        // cross the host link: a quarter fewer bytes per event.
+        // the copy engine wrote the event before the host published its
+        // descriptor, which thread 0 read with a system-scope acquire (that
+        // invalidates this compute unit's and this XCD's caches) before the
+        // barrier every wave passed since
+        //
+        // copy-in from the device arena, four 16-byte loads in flight per
+        // lane; the search never reads the hit IDs (N words after the sensor
+        // table), so only the words of the header and sensor table [0, h16)
+        // and those from the one holding X[0] on, [x16, n16), are copied
         const uint4* src = reinterpret_cast<const uint4*>(a.in_arena + in_off);
         uint4* dst = reinterpret_cast<uint4*>(blob);
         const int64_t n16 = bytes >> 4;
@@ -1898,15 +1927,32 @@ __device__ void serve_one(const ServeArgs& a, int w, Shared<B, C>& sh, uint64_t 
     __syncthreads();
     const uint64_t t_searched = wall_clock64();
     const EventResult r = a.base.results[w];
-    const bool spill = fits && r.n_tracks > out_cap;
-    if (fits && !spill) {
-        // copy-out into the request's pinned output region
-        const uint32_t* ts = reinterpret_cast<const uint32_t*>(a.base.tracks + (size_t)w * a.wg_cap);
-        uint32_t* to = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(a.out_arena) + out_off);
-        const int64_t words = (int64_t)r.n_tracks * (int64_t)(sizeof(clf_track) / 4);
-        for (int64_t k = tid; k < words; k += B) to[k] = ts[k];
+    const bool ok = fits && r.n_tracks >= 0 && r.n_tracks <= out_cap;
+    // copy-out into the device output ring: n_tracks records at a position
+    // reserved with one atomic on its allocator (the host's admission keeps
+    // the ring from overrunning what it has not copied back and released)
+    const int64_t words = ok ? (int64_t)r.n_tracks * (int64_t)(sizeof(clf_track) / 4) : 0;
+    if (tid == 0) {
+        unsigned long long* head = reinterpret_cast<unsigned long long*>(a.ctl + CTL_OUT_HEAD);
+        sh.pack_base = words > 0 ? (int64_t)__hip_atomic_fetch_add(head, (unsigned long long)(4 * words),
+                                                                   __ATOMIC_RELAXED,
+                                                                   __HIP_MEMORY_SCOPE_AGENT)
+                                 : 0;
     }
-    __threadfence_system();
+    __syncthreads();
+    const uint64_t pos = (uint64_t)uni(sh.pack_base);
+    if (words > 0) {
+        const uint32_t* ts = reinterpret_cast<const uint32_t*>(a.base.tracks + (size_t)w * a.wg_cap);
+        uint32_t* ring = reinterpret_cast<uint32_t*>(a.out_arena);
+        const int64_t ring_words = (int64_t)(a.out_bytes / 4);
+        const int64_t w0 = (int64_t)((pos / 4) % (uint64_t)ring_words);
+        const int64_t first = min(words, ring_words - w0);  // words before the ring's end
+        for (int64_t k = tid; k < first; k += B) ring[w0 + k] = ts[k];
+        for (int64_t k = first + tid; k < words; k += B) ring[k - first] = ts[k];
+    }
+    // every wave's ring stores are complete; thread 0's system-scope release
+    // of the completion below then makes them visible to the copy engine
+    // (one L2 writeback per event)
     __syncthreads();
     if (tid == 0) {
         const uint32_t p =
@@ -1914,13 +1960,15 @@ __device__ void serve_one(const ServeArgs& a, int w, Shared<B, C>& sh, uint64_t 
         CompRec* c = a.comp + (p & a.comp_mask);
         c->req = req;
         c->wg = w;
-        c->status = spill ? COMP_SPILL : COMP_OK;
-        c->res = r;
+        c->status = ok ? COMP_OK : COMP_INVALID;
+        EventResult rr = r;
+        if (!ok) rr.n_tracks = 0;
+        c->res = rr;
+        c->out_pos = pos;
         c->t_claim = t_claim;
         c->t_copied = t_copied;
         c->t_searched = t_searched;
         c->t_written = wall_clock64();
-        __threadfence_system();
         __hip_atomic_store(&c->seq, p + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
@@ -1943,11 +1991,12 @@ void serve_kernel(ServeArgs a) {
             uint32_t d = 0;
             const uint32_t pos = __hip_atomic_fetch_add(&ctl[CTL_IDLE_TAIL], 1u, __ATOMIC_RELAXED,
                                                         __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&a.idle_ring[pos & a.idle_mask],
-                               ((uint64_t)(pos + 1u) << 32) | (uint64_t)(uint32_t)w,
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint64_t* slot = &a.idle_ring[pos & a.idle_mask];
+            __hip_atomic_store(slot, idle_entry(pos, (uint32_t)w), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
             const uint64_t t0 = wall_clock64();
             int nap = 1;  // idle polls back off (64-cycle units, up to ~4k cycles)
+            bool claimed = false;  // the feeder took the entry: a hand-out is coming
             for (;;) {
                 uint32_t v = poll_agent(&a.mailbox[w]);
                 if (v == last_mb && poll_agent(&ctl[CTL_QUIT]) != 0u) {
@@ -1962,9 +2011,21 @@ void serve_kernel(ServeArgs a) {
                     got = 1;
                     break;
                 }
-                const uint64_t hb = __hip_atomic_load(beat, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const uint64_t since = hb > t0 ? hb : t0;
-                if ((int64_t)(wall_clock64() - since) > (int64_t)(4 * a.idle_ticks)) break;
+                if (!claimed) {
+                    const uint64_t hb = __hip_atomic_load(beat, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const uint64_t since = hb > t0 ? hb : t0;
+                    if ((int64_t)(wall_clock64() - since) > (int64_t)(4 * a.idle_ticks)) {
+                        // the feeder is silent: withdraw the entry and leave,
+                        // unless it has been claimed, in which case the
+                        // feeder writes the mailbox next
+                        uint64_t expect = idle_entry(pos, (uint32_t)w);
+                        if (__hip_atomic_compare_exchange_strong(
+                                slot, &expect, idle_entry(pos, kIdleGone), __ATOMIC_RELAXED,
+                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                            break;
+                        claimed = true;
+                    }
+                }
                 for (int k = 0; k < nap; ++k) __builtin_amdgcn_s_sleep(4);
                 nap = nap < 16 ? 2 * nap : 16;
             }
@@ -1978,7 +2039,6 @@ void serve_kernel(ServeArgs a) {
                 const uint4 q0 = reinterpret_cast<const uint4*>(sd)[0];
                 const uint4 q1 = reinterpret_cast<const uint4*>(sd)[1];
                 sh.sv_in = (uint64_t)q0.x | ((uint64_t)q0.y << 32);
-                sh.sv_out = (uint64_t)q0.z | ((uint64_t)q0.w << 32);
                 sh.sv_n = seen ? (int32_t)q1.x : -1;  // not seen cannot happen: the feeder saw it
                 sh.sv_s = (int32_t)q1.y;
                 sh.sv_req = (int32_t)q1.z;

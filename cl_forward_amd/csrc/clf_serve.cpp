@@ -5,12 +5,18 @@
 // engines take it from there:
 //
 // Resident: a persistent kernel (serve_kernel) follows a descriptor ring in
-// pinned memory.  Submit writes the descriptor and publishes it with a
-// release store of its sequence word; a workgroup claims it, reads the event
-// from the arena over the host link, searches it and writes the tracks into
-// the request's region of a pinned output arena, then appends a completion
-// record.  One host thread collects completions in order and relaunches the
-// kernel when it has left for lack of work.
+// pinned memory.  Submit copies the event into pinned staging; the server's
+// thread moves each run of newly submitted events into the device input
+// arena with one copy-engine transfer and, once it has landed, publishes
+// their descriptors (release store of each sequence word).  A workgroup
+// claims a descriptor, copies the event from the device arena into its
+// scratch, searches it, writes the tracks into the device output ring at a
+// position it reserves with one atomic, and appends a completion record.
+// One host thread collects completions, copies the finished, contiguous part
+// of the output ring back into its pinned mirror with the copy engine, hands
+// out the results, and relaunches the kernel when it has left for lack of
+// work.  Admission is by device memory: the input arena, and the output
+// ring's room for every admitted event's capacity bound.
 //
 // Batches: a dispatcher thread owns every HIP call.  Whenever one of the
 // slots (a stream with resident device buffers sized for a full batch) is
@@ -26,12 +32,18 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdio>
 #include <deque>
+#include <functional>
 #include <mutex>
 #include <numeric>
+#include <queue>
 #include <string>
 #include <thread>
 #include <vector>
+
+#include <hsa/hsa.h>
+#include <hsa/hsa_ext_amd.h>
 
 #include "clf_host.h"
 
@@ -50,6 +62,16 @@ constexpr int32_t kMaxSubmitMany = 64;  // events per clf_server_submit_many cal
 constexpr size_t kBlobAlign = 256;
 constexpr int64_t kEventBlobOverhead = 4096;  // header allowance per event in a slot
 constexpr int64_t kEventCapOverhead = 64 * 48 + 64;
+// Resident engine: copy-engine transfers in flight per direction.  While
+// they are, newly submitted (or finished) events accumulate and go in the
+// next transfer, so that the per-transfer host cost is spread over more
+// events the higher the load.
+constexpr size_t kCopiesInFlight = 2;
+// Output copies move whole 64 KiB-aligned pieces of the ring: a copy of a
+// few bytes, or at an odd offset, may be done with a kernel instead of a
+// copy engine, and such a kernel cannot start while the resident kernel
+// holds every compute unit.
+constexpr uint64_t kOutGrain = uint64_t(64) << 10;
 
 enum ReqState : int { R_FREE = 0, R_WRITING, R_PENDING, R_RUNNING, R_DONE };
 
@@ -99,17 +121,145 @@ struct Request {
     size_t span = 0;    // aligned blob bytes
     uint64_t in_seq = 0;
     bool in_live = false;
-    size_t out_off = 0;  // resident: its region in the output arena
-    uint64_t out_seq = 0;
     uint64_t desc = 0;   // resident: its descriptor number
     int32_t S = 0, N = 0;
     int64_t cap = 0;
     double cost = 0.0;
     int out_buf = -1;  // batches: output buffer holding its tracks once finished
-    int64_t out_cap = 0;             // resident: records its output region holds
-    std::vector<uint8_t> spill;      // resident: tracks of an event that overflowed it
+    // resident: the capacity bound in bytes, which the output ring reserves
+    // until completion; then its tracks' entry in the ring's live list
+    uint64_t out_cap_bytes = 0;
+    bool out_pending = false;  // admitted, not yet completed (counted in out_pend)
+    bool out_live = false;     // has an entry in the live list
+    uint64_t live_seq = 0;
     clf_served out{};
 };
+
+// A process-wide pool of fine-grained device buffers (coherent with the
+// copy engines while a kernel runs).  Buffers are reused by later servers
+// and never freed before the process ends.
+struct FinePool {
+    struct Buf {
+        int device;
+        void* p;
+        size_t bytes;
+        bool busy;
+    };
+    std::mutex mu;
+    std::vector<Buf> bufs;
+    hipError_t take(int device, size_t bytes, void** out) {
+        std::lock_guard<std::mutex> lk(mu);
+        for (Buf& b : bufs)
+            if (!b.busy && b.device == device && b.bytes >= bytes) {
+                b.busy = true;
+                *out = b.p;
+                return hipSuccess;
+            }
+        void* p = nullptr;
+        hipError_t e = hipExtMallocWithFlags(&p, bytes, hipDeviceMallocFinegrained);
+        if (e != hipSuccess) return e;
+        bufs.push_back({device, p, bytes, true});
+        *out = p;
+        return hipSuccess;
+    }
+    void give_back(void* p) {
+        std::lock_guard<std::mutex> lk(mu);
+        for (Buf& b : bufs)
+            if (b.p == p) b.busy = false;
+    }
+};
+FinePool g_fine;
+
+// Copy-engine transfers through the HSA runtime, with completion signals of
+// their own.  HIP streams share a few hardware queues per process, and a
+// transfer enqueued on a stream that shares the resident kernel's queue would
+// wait for the kernel to leave; these transfers are on no queue.
+struct Dma {
+    hsa_agent_t gpu{}, cpu{};
+    bool ready = false;
+    std::vector<hsa_signal_t> sigs;
+    std::vector<int> free_sigs;
+    std::mutex mu;
+};
+
+struct AgentSearch {
+    uint32_t domain, bus, dev;
+    Dma* dma;
+    bool found_gpu, found_cpu;
+};
+
+hsa_status_t visit_agent(hsa_agent_t agent, void* data) {
+    AgentSearch* a = static_cast<AgentSearch*>(data);
+    hsa_device_type_t type;
+    if (hsa_agent_get_info(agent, HSA_AGENT_INFO_DEVICE, &type) != HSA_STATUS_SUCCESS) return HSA_STATUS_SUCCESS;
+    if (type == HSA_DEVICE_TYPE_CPU && !a->found_cpu) {
+        a->dma->cpu = agent;
+        a->found_cpu = true;
+    } else if (type == HSA_DEVICE_TYPE_GPU && !a->found_gpu) {
+        uint32_t bdf = 0, domain = 0;
+        if (hsa_agent_get_info(agent, (hsa_agent_info_t)HSA_AMD_AGENT_INFO_BDFID, &bdf) == HSA_STATUS_SUCCESS &&
+            hsa_agent_get_info(agent, (hsa_agent_info_t)HSA_AMD_AGENT_INFO_DOMAIN, &domain) == HSA_STATUS_SUCCESS &&
+            domain == a->domain && ((bdf >> 8) & 0xffu) == a->bus && ((bdf >> 3) & 0x1fu) == a->dev) {
+            a->dma->gpu = agent;
+            a->found_gpu = true;
+        }
+    }
+    return HSA_STATUS_SUCCESS;
+}
+
+// Finds the HSA agents of HIP device `device` and of the host, and makes
+// `n_signals` completion signals.
+clf_status dma_open(Dma& d, int device, int n_signals) {
+    if (hsa_init() != HSA_STATUS_SUCCESS) return fail(CLF_E_DEVICE, "serving: hsa_init failed");
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    AgentSearch a{(uint32_t)prop.pciDomainID, (uint32_t)prop.pciBusID, (uint32_t)prop.pciDeviceID, &d, false,
+                  false};
+    (void)hsa_iterate_agents(visit_agent, &a);
+    if (!a.found_gpu || !a.found_cpu)
+        return fail(CLF_E_DEVICE, "serving: the device's copy-engine agent was not found");
+    d.sigs.resize((size_t)n_signals);
+    for (int k = 0; k < n_signals; ++k) {
+        if (hsa_signal_create(0, 0, nullptr, &d.sigs[(size_t)k]) != HSA_STATUS_SUCCESS)
+            return fail(CLF_E_OUT_OF_MEMORY, "serving: hsa_signal_create failed");
+        d.free_sigs.push_back(k);
+    }
+    d.ready = true;
+    return CLF_OK;
+}
+
+void dma_close(Dma& d) {
+    for (hsa_signal_t s : d.sigs)
+        if (s.handle) (void)hsa_signal_destroy(s);
+    d.sigs.clear();
+    d.free_sigs.clear();
+    if (d.ready) (void)hsa_shut_down();
+    d.ready = false;
+}
+
+int dma_take(Dma& d) {
+    std::lock_guard<std::mutex> lk(d.mu);
+    if (d.free_sigs.empty()) return -1;
+    const int k = d.free_sigs.back();
+    d.free_sigs.pop_back();
+    return k;
+}
+void dma_give(Dma& d, int k) {
+    std::lock_guard<std::mutex> lk(d.mu);
+    d.free_sigs.push_back(k);
+}
+
+clf_status hsa_fail(hsa_status_t e, const char* what) {
+    const char* msg = nullptr;
+    (void)hsa_status_string(e, &msg);
+    return fail(CLF_E_DEVICE, std::string(what) + ": " + (msg ? msg : "unknown HSA status"));
+}
+
+// 1: finished, 0: in flight, -1: failed.
+int dma_state(Dma& d, int k) {
+    const hsa_signal_value_t v = hsa_signal_load_scacquire(d.sigs[(size_t)k]);
+    return v == 0 ? 1 : (v > 0 ? 0 : -1);
+}
 
 // A slot's batch: RUNNING (copies in, search, stats out), DRAINED (its input
 // arena space returned, waiting for an output buffer), COPYING (tracks out).
@@ -168,11 +318,24 @@ struct clf_server {
     std::vector<OutBuf> outs;
 
     // resident engine
-    PinnedBuf out_arena, ring_mem, comp_mem, stop_mem;
-    Ring out_ring;
+    PinnedBuf ring_mem, comp_mem, stop_mem;  // coherent pinned: descriptors, completions, stop word
+    void* d_in = nullptr;                    // fine-grained device input arena (same offsets as `arena`)
+    void* d_out = nullptr;                   // fine-grained device output ring
+    PinnedBuf h_out;                         // pinned mirror of the output ring, plus out_ext bytes
+    uint64_t out_ring = 0, out_ext = 0;
+    // output ring accounting, absolute byte positions (lock held): completed
+    // bytes, capacity bytes of admitted requests not yet completed, and the
+    // released prefix
+    uint64_t out_done = 0, out_pend = 0, out_tail = 0;
+    struct Live {
+        uint64_t pos, size;
+        bool released;
+    };
+    std::deque<Live> live;  // ready allocations in position order, from out_tail on
+    uint64_t live_base = 0;
     uint32_t ring_mask = 0, comp_mask = 0;
     uint64_t next_desc = 0;
-    std::vector<uint8_t> desc_busy;
+    std::vector<int32_t> desc_req;  // request of each descriptor slot, -1 free
     std::atomic<uint64_t> published{0};
     uint64_t consumed = 0;
     uint32_t comp_next = 0;
@@ -181,10 +344,8 @@ struct clf_server {
     int grid = 0;
     uint32_t idle_mask = 0;
     int64_t wg_blob_bytes = 0, wg_hits = 0, wg_cap = 0;
-    // side lane for events whose tracks overflow their output region
-    Resident spill_dev;
-    PinnedBuf spill_desc;
-    hipStream_t spill_stream = nullptr;
+    // copies between the pinned and the device arenas, in both directions
+    Dma dma;
     hipStream_t rstream = nullptr;
     hipEvent_t ev_kernel = nullptr;
     bool kernel_running = false;
@@ -247,8 +408,16 @@ bool release_locked(clf_server* s, uint64_t ticket) {
         if (--o.outstanding == 0) o.busy = false;
     }
     r->out_buf = -1;
-    if (s->resident && r->out.status == CLF_OK) s->out_ring.release(r->out_seq);
-    std::vector<uint8_t>().swap(r->spill);
+    if (r->out_live) {
+        // its tracks' room in the output ring comes back from the front
+        s->live[r->live_seq - s->live_base].released = true;
+        while (!s->live.empty() && s->live.front().released) {
+            s->out_tail = s->live.front().pos + s->live.front().size;
+            s->live.pop_front();
+            ++s->live_base;
+        }
+        r->out_live = false;
+    }
     free_input(s, *r);
     r->ticket = 0;
     r->state.store(R_FREE);
@@ -373,8 +542,12 @@ clf_status poll_slot(clf_server* s, int k, bool& progress) {
         if (total < 0 || total > cap) return fail(CLF_E_DEVICE, "serving: inconsistent track count");
         sl.total = total;
         {
+            const int64_t t = now_ns();
             std::lock_guard<std::mutex> lk(s->mu);
-            for (int32_t idx : sl.reqs) free_input(s, s->reqs[idx]);
+            for (int32_t idx : sl.reqs) {
+                free_input(s, s->reqs[idx]);
+                s->reqs[idx].out.t_complete_ns = t;
+            }
         }
         sl.phase = S_DRAINED;
         progress = true;
@@ -516,12 +689,10 @@ ServeArgs serve_args(clf_server* s) {
     (void)hipHostGetDevicePointer(&p, s->ring_mem.p, 0);
     a.ring = static_cast<const ServeDesc*>(p);
     a.ring_mask = s->ring_mask;
-    (void)hipHostGetDevicePointer(&p, s->arena.p, 0);
-    a.in_arena = static_cast<const uint8_t*>(p);
-    (void)hipHostGetDevicePointer(&p, s->out_arena.p, 0);
-    a.out_arena = static_cast<clf_track*>(p);
+    a.in_arena = static_cast<const uint8_t*>(s->d_in);
+    a.out_arena = static_cast<uint8_t*>(s->d_out);
     a.in_bytes = s->arena.cap;
-    a.out_bytes = s->out_arena.cap;
+    a.out_bytes = s->out_ring;
     (void)hipHostGetDevicePointer(&p, s->comp_mem.p, 0);
     a.comp = static_cast<CompRec*>(p);
     a.comp_mask = s->comp_mask;
@@ -545,7 +716,7 @@ ServeArgs serve_args(clf_server* s) {
 clf_status launch_resident(clf_server* s) {
     // per-launch state: QUIT, IDLE_TAIL, BEAT, the mailboxes and the idle ring
     HIP_TRY(launch_zero_i32(s->ctl.as<int32_t>() + CTL_QUIT, 1, s->rstream));
-    HIP_TRY(launch_zero_i32(s->ctl.as<int32_t>() + CTL_IDLE_TAIL, CTL_WORDS - CTL_IDLE_TAIL, s->rstream));
+    HIP_TRY(launch_zero_i32(s->ctl.as<int32_t>() + CTL_IDLE_TAIL, CTL_OUT_HEAD - CTL_IDLE_TAIL, s->rstream));
     HIP_TRY(launch_zero_i32(s->mailbox.as<int32_t>(), s->grid, s->rstream));
     HIP_TRY(launch_zero_i32(s->idle_ring.as<int32_t>(), 2 * (int)(s->idle_mask + 1u), s->rstream));
     const ServeArgs a = serve_args(s);
@@ -557,39 +728,51 @@ clf_status launch_resident(clf_server* s) {
     return CLF_OK;
 }
 
-// Searches an event again on the side stream, for a request whose tracks
-// did not fit its output region; the tracks go to the request's own heap
-// buffer.  Blocking; the event is still in the input arena.
-clf_status search_spilled(clf_server* s, Request& r) {
-    EventDesc* d = s->spill_desc.as<EventDesc>();
-    d->blob_off = 0;
-    d->hit_off = 0;
-    d->track_off = 0;
-    d->n_hits = r.N;
-    d->n_sensors = r.S;
-    int32_t* order = reinterpret_cast<int32_t*>(d + 1);
-    order[0] = 0;
-    hipStream_t st = s->spill_stream;
-    const Resident& dev = s->spill_dev;
-    HIP_TRY(hipMemcpyAsync(dev.blobs.p, s->arena.as<uint8_t>() + r.in_off, event_bytes(r.S, r.N),
-                           hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(dev.desc.p, d, sizeof(EventDesc), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(dev.order.p, order, sizeof(int32_t), hipMemcpyHostToDevice, st));
-    clf_status cs = enqueue_search(dev, 1, CLF_MODE_BATCHED, s->cfg, s->n_cu, st, nullptr, nullptr);
-    if (cs != CLF_OK) return cs;
-    int64_t pre[2] = {0, 0};
-    HIP_TRY(hipMemcpyAsync(pre, dev.prefix.p, sizeof(pre), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    if (pre[1] < 0 || pre[1] > r.cap) return fail(CLF_E_DEVICE, "serving: inconsistent track count");
-    r.spill.resize(sizeof(clf_track) * (size_t)pre[1]);
-    if (pre[1] > 0)
-        HIP_TRY(hipMemcpy(r.spill.data(), dev.packed.p, r.spill.size(), hipMemcpyDeviceToHost));
-    return CLF_OK;
+// Ends a resident request (lock held): its result is ready, or it failed.
+void finish_resident(clf_server* s, int32_t idx, clf_status st, const EventResult* res,
+                     const uint8_t* tracks, int64_t t_done) {
+    Request& r = s->reqs[idx];
+    free_input(s, r);
+    s->desc_req[r.desc & s->ring_mask] = -1;
+    r.out.ticket = r.ticket;
+    r.out.tag = r.tag;
+    r.out.status = st;
+    r.out.batch_events = 1;
+    if (res) std::memcpy(&r.out.stats, res, sizeof(clf_event_stats));
+    r.out.n_tracks = st == CLF_OK && res ? res->n_tracks : 0;
+    r.out.tracks = st == CLF_OK ? tracks : nullptr;
+    r.out.t_done_ns = t_done;
+    r.state.store(R_DONE);
+    if (s->collect) s->completed.push_back(idx);
 }
 
-// Takes finished requests off the completion ring, in order: copies what is
-// there, then updates the requests under one lock.
-bool collect_completions(clf_server* s, std::vector<CompRec>& buf) {
+// The resident engine's view of the output ring on the completion thread:
+// completed allocations not yet contiguous with the copied part, the
+// contiguous end, what has been copied back, and the copies in flight.
+struct OutCopy {
+    struct Done {
+        uint64_t pos, size;
+        int32_t req;
+        EventResult res;
+        bool operator>(const Done& o) const { return pos > o.pos; }
+    };
+    std::priority_queue<Done, std::vector<Done>, std::greater<Done>> waiting;
+    uint64_t ready = 0;  // every allocation below is complete
+    std::vector<Done> batch;  // ready, not yet in a copy
+    struct InFlight {
+        int sig;
+        std::vector<Done> reqs;
+    };
+    std::deque<InFlight> flights;
+    // the signal of the last copy issued (the next one waits for it), and
+    // of the last one finished (kept until its successor has finished too,
+    // since that one may still be waiting on it)
+    int last_issued = -1, last_done = -1;
+};
+
+// Takes finished requests off the completion ring, in order, and moves the
+// output ring's completed prefix toward the host.
+bool collect_completions(clf_server* s, std::vector<CompRec>& buf, OutCopy& oc) {
     CompRec* ring = s->comp_mem.as<CompRec>();
     buf.clear();
     for (;;) {
@@ -600,70 +783,249 @@ bool collect_completions(clf_server* s, std::vector<CompRec>& buf) {
         if (buf.size() >= 4096) break;
     }
     if (buf.empty()) return false;
-    std::vector<int32_t> spilled;
     const int64_t t = now_ns();
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        for (const CompRec& rec : buf) {
-            ++s->consumed;
-            ++s->n_batched_events;
-            s->dev_in += rec.t_copied - rec.t_claim;
-            s->dev_search += rec.t_searched - rec.t_copied;
-            s->dev_out += rec.t_written - rec.t_searched;
-            ++s->dev_n;
-            if (rec.req < 0 || (size_t)rec.req >= s->reqs.size()) continue;
-            Request& r = s->reqs[rec.req];
-            if (r.state.load() != R_PENDING) continue;  // already ended by a failure
-            if (rec.status == COMP_SPILL) {
-                std::memcpy(&r.out.stats, &rec.res, sizeof(clf_event_stats));
-                spilled.push_back(rec.req);
-                continue;
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (const CompRec& rec : buf) {
+        ++s->consumed;
+        ++s->n_batched_events;
+        s->dev_in += rec.t_copied - rec.t_claim;
+        s->dev_search += rec.t_searched - rec.t_copied;
+        s->dev_out += rec.t_written - rec.t_searched;
+        ++s->dev_n;
+        if (rec.req < 0 || (size_t)rec.req >= s->reqs.size()) continue;
+        Request& r = s->reqs[rec.req];
+        if (r.state.load() != R_PENDING) continue;  // already ended by a failure
+        r.out.t_complete_ns = t;
+        const uint64_t size = (uint64_t)std::max(rec.res.n_tracks, 0) * sizeof(clf_track);
+        if (r.out_pending) {
+            s->out_pend -= r.out_cap_bytes;
+            r.out_pending = false;
+        }
+        if (rec.status != COMP_OK) {
+            finish_resident(s, rec.req, CLF_E_DEVICE, &rec.res, nullptr, t);
+            continue;
+        }
+        s->out_done += size;
+        if (size == 0) {
+            finish_resident(s, rec.req, CLF_OK, &rec.res, s->h_out.as<uint8_t>(), t);
+            continue;
+        }
+        oc.waiting.push({rec.out_pos, size, rec.req, rec.res});
+    }
+    // allocations are contiguous: the next ready one starts where the last ended
+    while (!oc.waiting.empty() && oc.waiting.top().pos == oc.ready) {
+        const OutCopy::Done d = oc.waiting.top();
+        oc.waiting.pop();
+        oc.ready = d.pos + d.size;
+        s->live.push_back({d.pos, d.size, false});
+        Request& r = s->reqs[d.req];
+        r.out_live = true;
+        r.live_seq = s->live_base + s->live.size() - 1;
+        oc.batch.push_back(d);
+    }
+    return true;
+}
+
+// Copies the batch's part of the output ring [batch[0].pos, ready) into the
+// pinned mirror at the same offsets (an allocation that runs past the
+// ring's end is also copied past the mirror's end, so that its tracks are
+// contiguous there); results are handed out when the copy has finished.
+// Each piece is widened to kOutGrain boundaries.  That is safe: below the
+// batch the device still holds the bytes already copied (live results are
+// never overwritten, by the admission rule), and above it lie positions
+// that no live result occupies in the mirror, which their own copies
+// overwrite later (the copies run in order on one stream).
+clf_status copy_back(clf_server* s, OutCopy& oc) {
+    if (oc.batch.empty() || oc.flights.size() >= kCopiesInFlight) return CLF_OK;
+    const int k = dma_take(s->dma);
+    if (k < 0) return CLF_OK;  // every signal in use: next round
+    const uint64_t R = s->out_ring, G = kOutGrain;
+    const uint64_t lo = oc.batch.front().pos, hi = oc.ready;
+    uint8_t* dev = static_cast<uint8_t*>(s->d_out);
+    uint8_t* host = s->h_out.as<uint8_t>();
+    struct Piece {
+        uint8_t* dst;
+        uint64_t a, b;
+    };
+    Piece pieces[8];
+    int np = 0;
+    auto piece = [&](uint8_t* dst, uint64_t from, uint64_t to, uint64_t limit) {
+        pieces[np++] = {dst, from / G * G, std::min(limit, (to + G - 1) / G * G)};
+    };
+    const uint64_t a = lo % R, len = hi - lo;
+    if (a + len <= R) {
+        piece(host, a, a + len, R);
+    } else {
+        piece(host, a, R, R);
+        piece(host, 0, a + len - R, R);
+    }
+    for (const OutCopy::Done& d : oc.batch) {
+        const uint64_t b0 = d.pos % R;
+        if (b0 + d.size > R) {  // this one wraps: its tail again past the mirror's end
+            if (b0 + d.size - R > s->out_ext) {
+                dma_give(s->dma, k);
+                return fail(CLF_E_DEVICE, "serving: output larger than the mirror");
             }
-            free_input(s, r);
-            s->desc_busy[r.desc & s->ring_mask] = 0;
-            r.out.ticket = r.ticket;
-            r.out.tag = r.tag;
-            r.out.status = CLF_OK;
-            r.out.batch_events = 1;
-            std::memcpy(&r.out.stats, &rec.res, sizeof(clf_event_stats));
-            r.out.n_tracks = rec.res.n_tracks;
-            r.out.tracks = s->out_arena.as<uint8_t>() + r.out_off;
-            r.out.t_launch_ns = 0;
-            r.out.t_done_ns = t;
-            r.state.store(R_DONE);
-            if (s->collect) s->completed.push_back(rec.req);
+            if (np < 8) piece(host + R, 0, b0 + d.size - R, s->out_ext);
         }
     }
-    for (int32_t idx : spilled) {
-        Request& r = s->reqs[idx];
-        const clf_status st = search_spilled(s, r);
-        std::lock_guard<std::mutex> lk(s->mu);
-        free_input(s, r);
-        s->desc_busy[r.desc & s->ring_mask] = 0;
-        r.out.ticket = r.ticket;
-        r.out.tag = r.tag;
-        r.out.status = st;
-        r.out.batch_events = 1;
-        r.out.n_tracks = st == CLF_OK ? (int64_t)(r.spill.size() / sizeof(clf_track)) : 0;
-        r.out.tracks = st == CLF_OK ? r.spill.data() : nullptr;
-        r.out.t_launch_ns = 0;
-        r.out.t_done_ns = now_ns();
-        r.state.store(R_DONE);
-        if (s->collect) s->completed.push_back(idx);
+    // in order after the previous copy (the widened pieces of consecutive
+    // copies overlap, and the later one must land last)
+    hsa_signal_t sig = s->dma.sigs[(size_t)k];
+    hsa_signal_store_relaxed(sig, np);
+    const int ndep = oc.last_issued >= 0 ? 1 : 0;
+    hsa_signal_t dep = ndep ? s->dma.sigs[(size_t)oc.last_issued] : hsa_signal_t{0};
+    for (int i = 0; i < np; ++i) {
+        const hsa_status_t e = hsa_amd_memory_async_copy(pieces[i].dst + pieces[i].a, s->dma.cpu, dev + pieces[i].a,
+                                                         s->dma.gpu, pieces[i].b - pieces[i].a, ndep,
+                                                         ndep ? &dep : nullptr, sig);
+        if (e != HSA_STATUS_SUCCESS) {
+            char what[256];
+            std::snprintf(what, sizeof(what), "serving: output copy could not be issued (dst %p src %p bytes %llu, %d deps)",
+                          (void*)(pieces[i].dst + pieces[i].a), (void*)(dev + pieces[i].a),
+                          (unsigned long long)(pieces[i].b - pieces[i].a), ndep);
+            return hsa_fail(e, what);
+        }
     }
-    s->cv_done.notify_all();
-    if (s->collect) s->cv_completed.notify_all();
-    return true;
+    oc.last_issued = k;
+    oc.flights.push_back({k, std::move(oc.batch)});
+    oc.batch.clear();
+    return CLF_OK;
+}
+
+// Hands out the results of finished copies, oldest first.
+clf_status finish_copies(clf_server* s, OutCopy& oc, bool& any) {
+    any = false;
+    while (!oc.flights.empty()) {
+        const int st = dma_state(s->dma, oc.flights.front().sig);
+        if (st == 0) break;
+        if (st < 0) return fail(CLF_E_DEVICE, "serving: output copy failed");
+        OutCopy::InFlight f = std::move(oc.flights.front());
+        oc.flights.pop_front();
+        if (oc.last_done >= 0) dma_give(s->dma, oc.last_done);
+        oc.last_done = f.sig;
+        if (oc.last_issued == f.sig && oc.flights.empty()) {
+            // nothing waits on it any more, and the next copy need not
+            oc.last_issued = -1;
+            dma_give(s->dma, f.sig);
+            oc.last_done = -1;
+        }
+        const int64_t t = now_ns();
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (const OutCopy::Done& d : f.reqs) {
+            if (s->reqs[d.req].state.load() != R_PENDING) continue;
+            finish_resident(s, d.req, CLF_OK, &d.res, s->h_out.as<uint8_t>() + d.pos % s->out_ring, t);
+        }
+        any = true;
+    }
+    return CLF_OK;
+}
+
+// The upload step of the resident loop: runs of newly submitted events (in
+// descriptor order, each already copied into the staging arena by its
+// submitter) go to the device input arena in one transfer per contiguous
+// run, at most kCopiesInFlight transfers at a time; when a transfer has
+// finished, the descriptors of its events are published.
+struct Upload {
+    uint64_t next = 0;       // first descriptor not yet sent
+    uint64_t published = 0;  // first descriptor not yet published
+    struct Sent {
+        int sig;
+        uint64_t d_end;
+    };
+    std::deque<Sent> sent;
+    struct Run {
+        size_t off, len;
+    };
+    std::vector<Run> runs;
+};
+
+clf_status upload_step(clf_server* s, Upload& u, bool& progress) {
+    // transfers that have finished: publish their descriptors
+    while (!u.sent.empty()) {
+        const int st = dma_state(s->dma, u.sent.front().sig);
+        if (st == 0) break;
+        if (st < 0) return fail(CLF_E_DEVICE, "serving: input copy failed");
+        const uint64_t end = u.sent.front().d_end;
+        dma_give(s->dma, u.sent.front().sig);
+        u.sent.pop_front();
+        const int64_t t = now_ns();
+        ServeDesc* ring = s->ring_mem.as<ServeDesc>();
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            for (; u.published < end; ++u.published) {
+                const int32_t idx = s->desc_req[u.published & s->ring_mask];
+                if (idx < 0 || s->reqs[idx].state.load() != R_PENDING) continue;  // ended by a failure
+                Request& r = s->reqs[idx];
+                ServeDesc* sd = ring + (u.published & s->ring_mask);
+                sd->in_off = r.in_off;
+                sd->reserved = 0;
+                sd->n_hits = r.N;
+                sd->n_sensors = r.S;
+                sd->req = idx;
+                sd->out_cap = (int32_t)CLF_EVENT_TRACK_CAPACITY(r.N, r.S);
+                r.out.t_launch_ns = t;
+                __atomic_store_n(&sd->seq, (uint32_t)(u.published + 1), __ATOMIC_RELEASE);
+            }
+        }
+        s->published.store(u.published);
+        progress = true;
+    }
+    if (u.sent.size() >= kCopiesInFlight) return CLF_OK;
+    // the newly submitted events whose staging copies are complete
+    uint64_t d_end = u.next;
+    u.runs.clear();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        while (d_end < s->next_desc && u.runs.size() < 64) {
+            const int32_t idx = s->desc_req[d_end & s->ring_mask];
+            if (idx < 0) break;
+            const Request& r = s->reqs[idx];
+            if (r.desc != d_end || r.state.load(std::memory_order_acquire) != R_PENDING) break;
+            if (!u.runs.empty() && u.runs.back().off + u.runs.back().len == r.in_off)
+                u.runs.back().len += r.span;
+            else
+                u.runs.push_back({r.in_off, r.span});
+            ++d_end;
+        }
+    }
+    if (d_end == u.next) return CLF_OK;
+    const int k = dma_take(s->dma);
+    if (k < 0) return CLF_OK;
+    hsa_signal_t sig = s->dma.sigs[(size_t)k];
+    hsa_signal_store_relaxed(sig, (hsa_signal_value_t)u.runs.size());
+    for (const Upload::Run& run : u.runs) {
+        const hsa_status_t e = hsa_amd_memory_async_copy(static_cast<uint8_t*>(s->d_in) + run.off, s->dma.gpu,
+                                                         s->arena.as<uint8_t>() + run.off, s->dma.cpu, run.len,
+                                                         0, nullptr, sig);
+        if (e != HSA_STATUS_SUCCESS) return hsa_fail(e, "serving: input copy could not be issued");
+    }
+    u.sent.push_back({k, d_end});
+    u.next = d_end;
+    progress = true;
+    return CLF_OK;
 }
 
 void resident_loop(clf_server* s) {
     (void)hipSetDevice(s->device);
     std::vector<CompRec> buf;
     buf.reserve(4096);
+    OutCopy oc;
+    Upload up;
     int64_t last_query = 0;
     for (;;) {
-        bool progress = collect_completions(s, buf);
-        clf_status st = CLF_OK;
+        bool progress = false;
+        clf_status st = upload_step(s, up, progress);
+        progress = collect_completions(s, buf, oc) || progress;
+        bool copied = false;
+        if (st == CLF_OK) st = finish_copies(s, oc, copied);
+        if (st == CLF_OK) st = copy_back(s, oc);
+        progress = progress || copied;
+        if (copied || progress) {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->cv_done.notify_all();
+            if (s->collect) s->cv_completed.notify_all();
+        }
         bool stopping, outstanding;
         {
             std::lock_guard<std::mutex> lk(s->mu);
@@ -672,12 +1034,12 @@ void resident_loop(clf_server* s) {
         }
         // has the kernel left?  (asked at most every 100 us while it has work)
         const int64_t now = now_ns();
-        if (s->kernel_running && (!outstanding || stopping || now - last_query > 100000)) {
+        if (st == CLF_OK && s->kernel_running && (!outstanding || stopping || now - last_query > 100000)) {
             last_query = now;
             hipError_t q = hipEventQuery(s->ev_kernel);
             if (q == hipSuccess) {
                 s->kernel_running = false;
-                progress = collect_completions(s, buf) || progress;
+                progress = collect_completions(s, buf, oc) || progress;
                 std::lock_guard<std::mutex> lk(s->mu);
                 outstanding = s->consumed < s->published.load();
             } else if (q != hipErrorNotReady) {
@@ -693,14 +1055,20 @@ void resident_loop(clf_server* s) {
             fail_all(s, st, std::string("serving: ") + clf_last_error());
             __atomic_store_n(s->stop_mem.as<uint32_t>(), 1u, __ATOMIC_RELEASE);
         }
-        if ((stopping || st != CLF_OK) && !s->kernel_running) return;
+        const bool copies_left = !oc.flights.empty() || !oc.batch.empty() || !up.sent.empty();
+        if ((stopping || st != CLF_OK) && !s->kernel_running && !copies_left) return;
         if (st != CLF_OK) {
             (void)hipStreamSynchronize(s->rstream);
+            // let the copies in flight end before their buffers can go
+            for (const Upload::Sent& x : up.sent)
+                while (dma_state(s->dma, x.sig) == 0) std::this_thread::yield();
+            for (const OutCopy::InFlight& x : oc.flights)
+                while (dma_state(s->dma, x.sig) == 0) std::this_thread::yield();
             s->kernel_running = false;
             return;
         }
         if (progress) continue;
-        if (s->kernel_running || outstanding) {
+        if (s->kernel_running || outstanding || copies_left) {
             std::this_thread::yield();
             continue;
         }
@@ -724,13 +1092,20 @@ clf_status create_resident(clf_server* s) {
     while (comp_n < c.max_requests) comp_n <<= 1;
     s->ring_mask = (uint32_t)ring_n - 1u;
     s->comp_mask = (uint32_t)comp_n - 1u;
-    s->desc_busy.assign((size_t)ring_n, 0);
+    s->desc_req.assign((size_t)ring_n, -1);
+    // the output ring (a multiple of kOutGrain) and its pinned mirror, which
+    // has room past its end for the largest event's tracks
+    s->out_ring = (uint64_t)c.out_arena_bytes / kOutGrain * kOutGrain;
+    s->out_ext = ((uint64_t)CLF_EVENT_TRACK_CAPACITY(s->wg_hits, CLF_MAX_SENSORS) * sizeof(clf_track) +
+                  kOutGrain - 1) / kOutGrain * kOutGrain;
     const unsigned coh = hipHostMallocCoherent | hipHostMallocMapped;
     hipError_t e = hipSuccess;
     auto chk = [&](hipError_t r) {
         if (e == hipSuccess) e = r;
     };
-    chk(s->out_arena.ensure((size_t)c.out_arena_bytes, coh));
+    chk(g_fine.take(s->device, s->arena.cap, &s->d_in));
+    chk(g_fine.take(s->device, (size_t)s->out_ring, &s->d_out));
+    chk(s->h_out.ensure((size_t)(s->out_ring + s->out_ext)));
     chk(s->ring_mem.ensure(sizeof(ServeDesc) * (size_t)ring_n, coh));
     chk(s->comp_mem.ensure(sizeof(CompRec) * (size_t)comp_n, coh));
     chk(s->stop_mem.ensure(256, coh));
@@ -740,9 +1115,6 @@ clf_status create_resident(clf_server* s) {
     s->idle_mask = (uint32_t)idle_n - 1u;
     chk(s->mailbox.ensure(sizeof(uint32_t) * g));
     chk(s->idle_ring.ensure(sizeof(uint64_t) * (size_t)idle_n));
-    chk(s->spill_dev.ensure(1, (size_t)s->wg_blob_bytes, s->wg_hits, s->wg_cap));
-    chk(s->spill_desc.ensure(256));
-    chk(hipStreamCreateWithFlags(&s->spill_stream, hipStreamNonBlocking));
     chk(s->wg_blobs.ensure(g * (size_t)s->wg_blob_bytes));
     chk(s->wg_desc.ensure(sizeof(EventDesc) * g));
     chk(s->wg_cand.ensure(sizeof(int4) * g * (size_t)s->wg_hits));
@@ -755,12 +1127,13 @@ clf_status create_resident(clf_server* s) {
     chk(hipStreamCreateWithFlags(&s->rstream, hipStreamNonBlocking));
     chk(hipEventCreateWithFlags(&s->ev_kernel, hipEventDisableTiming));
     if (e != hipSuccess) return hip_fail(e, "resident server allocation");
+    clf_status ds = dma_open(s->dma, s->device, 64);
+    if (ds != CLF_OK) return ds;
     // touch every page once so that no request pays for it
-    std::memset(s->out_arena.p, 0, s->out_arena.cap);
+    std::memset(s->h_out.p, 0, s->h_out.cap);
     std::memset(s->ring_mem.p, 0, s->ring_mem.cap);
     std::memset(s->comp_mem.p, 0, s->comp_mem.cap);
     std::memset(s->stop_mem.p, 0, s->stop_mem.cap);
-    s->out_ring.cap = s->out_arena.cap;
     HIP_TRY(launch_zero_i32(s->ctl.as<int32_t>(), CTL_WORDS, s->rstream));
     HIP_TRY(hipStreamSynchronize(s->rstream));
     return CLF_OK;
@@ -802,8 +1175,9 @@ clf_status submit_events(clf_server* s, int32_t n, const uint8_t* const* events,
                          const uint64_t* tags, uint64_t* tickets, int32_t* accepted) {
     struct Sized {
         int32_t S, N;
-        size_t bytes, span, out_span;
-        int64_t cap, out_cap;
+        size_t bytes, span;
+        int64_t cap;
+        uint64_t out_cap_bytes;
     };
     Sized z[kMaxSubmitMany];
     Request* rq[kMaxSubmitMany];
@@ -820,11 +1194,10 @@ clf_status submit_events(clf_server* s, int32_t n, const uint8_t* const* events,
         e.bytes = event_bytes(e.S, e.N);
         e.span = align_up(e.bytes, kBlobAlign);
         e.cap = track_slice(e.N, e.S);
-        e.out_cap = std::min<int64_t>(e.cap, (int64_t)s->conf.out_min_tracks +
-                                                 (int64_t)e.N * s->conf.out_tracks_per_1k_hits / 1000);
-        e.out_span = align_up(sizeof(clf_track) * (size_t)std::max<int64_t>(e.out_cap, 1), kBlobAlign);
+        e.out_cap_bytes = (uint64_t)CLF_EVENT_TRACK_CAPACITY(e.N, e.S) * sizeof(clf_track);
         if (s->resident) {
-            if (e.N > s->wg_hits || e.cap > s->wg_cap || (int64_t)e.span > s->wg_blob_bytes) {
+            if (e.N > s->wg_hits || e.cap > s->wg_cap || (int64_t)e.span > s->wg_blob_bytes ||
+                e.out_cap_bytes > s->out_ring) {
                 refused = CLF_E_CAPACITY, why = "event larger than the serving kernel's scratch";
                 break;
             }
@@ -853,7 +1226,7 @@ clf_status submit_events(clf_server* s, int32_t n, const uint8_t* const* events,
                 refused = CLF_E_CAPACITY, why = "request table full";
                 break;
             }
-            size_t off, taken, ooff = 0, otaken = 0;
+            size_t off, taken;
             if (!s->in_ring.fits(e.span, off, taken)) {
                 refused = CLF_E_CAPACITY, why = "input arena full";
                 break;
@@ -861,12 +1234,15 @@ clf_status submit_events(clf_server* s, int32_t n, const uint8_t* const* events,
             uint64_t d = 0;
             if (s->resident) {
                 d = s->next_desc;
-                if (s->desc_busy[d & s->ring_mask]) {
+                if (s->desc_req[d & s->ring_mask] >= 0) {
                     refused = CLF_E_CAPACITY, why = "descriptor ring full";
                     break;
                 }
-                if (!s->out_ring.fits(e.out_span, ooff, otaken)) {
-                    refused = CLF_E_CAPACITY, why = "output arena full";
+                // the output ring holds every admitted event's capacity bound
+                // beyond what completed events have used, from its released
+                // prefix on, so a worker's reservation never overruns it
+                if (s->out_done + s->out_pend + e.out_cap_bytes > s->out_tail + s->out_ring) {
+                    refused = CLF_E_CAPACITY, why = "device output ring full";
                     break;
                 }
             }
@@ -880,16 +1256,17 @@ clf_status submit_events(clf_server* s, int32_t n, const uint8_t* const* events,
             r->in_seq = s->in_ring.commit(off, e.span, taken);
             r->in_live = true;
             if (s->resident) {
-                r->out_off = ooff;
-                r->out_seq = s->out_ring.commit(ooff, e.out_span, otaken);
                 r->desc = d;
-                s->desc_busy[d & s->ring_mask] = 1;
+                s->desc_req[d & s->ring_mask] = idx;
                 ++s->next_desc;
+                r->out_cap_bytes = e.out_cap_bytes;
+                r->out_pending = true;
+                r->out_live = false;
+                s->out_pend += e.out_cap_bytes;
             }
             r->S = e.S;
             r->N = e.N;
             r->cap = e.cap;
-            r->out_cap = e.out_cap;
             r->out_buf = -1;
             r->out = clf_served{};
             r->state.store(R_WRITING);
@@ -898,31 +1275,18 @@ clf_status submit_events(clf_server* s, int32_t n, const uint8_t* const* events,
             rq[taken_n] = r;
         }
     }
-    int32_t published = 0;
+    (void)d0;
     for (int32_t k = 0; k < taken_n; ++k) {
         Request* r = rq[k];
         const Sized& e = z[k];
         std::memcpy(s->arena.as<uint8_t>() + r->in_off, events[k], e.bytes);
         r->cost = event_cost(events[k], e.S, e.N);
         r->out.t_submit_ns = now_ns();
-        // a failure of the server may have ended the request meanwhile; its
-        // descriptor is then never read (the kernel has been told to stop)
+        // a failure of the server may have ended the request meanwhile; it
+        // is then never uploaded (the kernel has been told to stop)
         int expected = R_WRITING;
-        if (!r->state.compare_exchange_strong(expected, R_PENDING, std::memory_order_acq_rel)) continue;
-        if (s->resident) {
-            const uint64_t d = d0 + (uint64_t)k;
-            ServeDesc* sd = s->ring_mem.as<ServeDesc>() + (d & s->ring_mask);
-            sd->in_off = r->in_off;
-            sd->out_off = r->out_off;
-            sd->n_hits = e.N;
-            sd->n_sensors = e.S;
-            sd->req = (int32_t)(r - s->reqs.data());
-            sd->out_cap = (int32_t)e.out_cap;
-            __atomic_store_n(&sd->seq, (uint32_t)(d + 1), __ATOMIC_RELEASE);
-            ++published;
-        }
+        (void)r->state.compare_exchange_strong(expected, R_PENDING, std::memory_order_acq_rel);
     }
-    if (published > 0) s->published.fetch_add((uint64_t)published);
     *accepted = taken_n;
     if (taken_n > 0) s->cv_work.notify_one();
     if (refused != CLF_OK) return why ? fail(refused, why) : fail(refused, err_msg);
@@ -938,15 +1302,13 @@ void clf_server_default_config(clf_server_config* cfg) {
     cfg->engine = CLF_SERVE_RESIDENT;
     cfg->max_requests = 65536;
     cfg->arena_bytes = (int64_t)512 << 20;
-    cfg->out_arena_bytes = (int64_t)1 << 30;
+    cfg->out_arena_bytes = (int64_t)1 << 30;  // device output ring
     cfg->max_event_hits = 32768;
     cfg->idle_us = 5000;
     cfg->slots = 4;
     cfg->max_batch_events = 256;
     cfg->max_batch_hits = 1 << 20;
     cfg->max_wait_us = 100;
-    cfg->out_tracks_per_1k_hits = 500;
-    cfg->out_min_tracks = 256;
     cfg->reserved = 0;
 }
 
@@ -980,17 +1342,16 @@ void clf_server_destroy(clf_server* s) {
     }
     for (OutBuf& o : s->outs) o.mem.release();
     if (s->rstream) (void)hipStreamSynchronize(s->rstream);
-    if (s->spill_stream) (void)hipStreamSynchronize(s->spill_stream);
     DevBuf* dev[] = {&s->ctl,       &s->mailbox,  &s->idle_ring, &s->wg_blobs,
                      &s->wg_desc,   &s->wg_cand,  &s->wg_used,   &s->wg_tracks,
                      &s->wg_ttf_a,  &s->wg_ttf_b, &s->wg_weak,   &s->wg_results};
     for (DevBuf* d : dev) d->release();
-    s->spill_dev.release();
-    s->spill_desc.release();
-    if (s->spill_stream) (void)hipStreamDestroy(s->spill_stream);
+    if (s->d_in) g_fine.give_back(s->d_in);
+    if (s->d_out) g_fine.give_back(s->d_out);
+    dma_close(s->dma);
     if (s->ev_kernel) (void)hipEventDestroy(s->ev_kernel);
     if (s->rstream) (void)hipStreamDestroy(s->rstream);
-    s->out_arena.release();
+    s->h_out.release();
     s->ring_mem.release();
     s->comp_mem.release();
     s->stop_mem.release();
@@ -1009,8 +1370,7 @@ clf_status clf_server_create(clf_context* ctx, const clf_server_config* cfg, clf
         return fail(CLF_E_INVALID_ARGUMENT, "bad server configuration");
     if (c.engine == CLF_SERVE_RESIDENT &&
         (c.max_event_hits < 1 || c.max_event_hits > CLF_MAX_SENSOR_HITS * 64 || c.idle_us < 1 ||
-         c.idle_us > 10000000 || c.out_arena_bytes < (int64_t)(1 << 20) ||
-         c.out_tracks_per_1k_hits < 0 || c.out_min_tracks < 0))
+         c.idle_us > 10000000 || c.out_arena_bytes < (int64_t)(1 << 20)))
         return fail(CLF_E_INVALID_ARGUMENT, "bad resident server configuration");
     if (c.engine == CLF_SERVE_BATCHES &&
         (c.slots < 1 || c.slots > kMaxSlots || c.max_batch_events < 1 ||
@@ -1036,9 +1396,7 @@ clf_status clf_server_create(clf_context* ctx, const clf_server_config* cfg, clf
 
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess)
-        e = s->arena.ensure((size_t)c.arena_bytes,
-                            s->resident ? hipHostMallocCoherent | hipHostMallocMapped
-                                        : hipHostMallocDefault);
+        e = s->arena.ensure((size_t)c.arena_bytes);  // staging: read by the copy engines only
     if (e != hipSuccess) {
         clf_server_destroy(s);
         return hip_fail(e, "server arena");
@@ -1095,6 +1453,9 @@ clf_status clf_server_wait(clf_server* s, uint64_t ticket, int32_t timeout_ms, c
         return fail(CLF_E_CAPACITY, "timed out");
     }
     *out = r->out;
+    // a request ended by a server failure carries the failure's message
+    if (out->status != CLF_OK && s->error == out->status && !s->error_msg.empty())
+        (void)fail((clf_status)out->status, s->error_msg);
     return CLF_OK;
 }
 
@@ -1147,7 +1508,7 @@ clf_status clf_server_replay(clf_server* s, int32_t n_pool, const uint8_t* const
     for (int64_t i = 0; i < n; ++i) {
         if (pick[i] < 0 || pick[i] >= n_pool) return fail(CLF_E_INVALID_ARGUMENT, "pick out of range");
         if (arrival_ns[i] < 0) return fail(CLF_E_INVALID_ARGUMENT, "negative arrival time");
-        rec[i] = clf_replay_record{-1, -1, 0, 0, -1, 0, 0};
+        rec[i] = clf_replay_record{-1, -1, 0, 0, -1, 0, 0, -1};
     }
     {
         std::lock_guard<std::mutex> lk(s->mu);
@@ -1240,6 +1601,7 @@ clf_status clf_server_replay(clf_server* s, int32_t n_pool, const uint8_t* const
                 r.n_tracks = (int32_t)o.n_tracks;
                 r.flags = o.stats.flags;
                 r.batch_events = o.batch_events;
+                r.device_ns = o.t_complete_ns > 0 && o.t_launch_ns > 0 ? o.t_complete_ns - o.t_launch_ns : -1;
                 r.digest = o.status == CLF_OK ? clf_tracks_digest(o.tracks, o.n_tracks) : 0;
                 ld_max = std::max(ld_max, o.t_done_ns);
             }
@@ -1293,8 +1655,11 @@ clf_status clf_server_replay(clf_server* s, int32_t n_pool, const uint8_t* const
         sum->max_ns = lat.empty() ? 0 : lat.back();
         std::lock_guard<std::mutex> lk(s->mu);
         sum->batches = s->n_batches - b0;
-        sum->mean_batch_events =
-            sum->batches > 0 ? (double)(s->n_batched_events - e0) / (double)sum->batches : 0.0;
+        if (s->resident)
+            sum->mean_batch_events = sum->completed > 0 ? 1.0 : 0.0;
+        else
+            sum->mean_batch_events =
+                sum->batches > 0 ? (double)(s->n_batched_events - e0) / (double)sum->batches : 0.0;
         sum->max_submit_lag_ns = (double)max_lag.load();
         const double dn = (double)(s->dev_n - dn0);
         const double tk = s->tick_ns * 1e-3;

@@ -220,20 +220,25 @@ clf_status clf_probe_link(clf_context* ctx, int64_t bytes, int32_t reps, double*
  * the same event.  Two engines:
  *
  *  CLF_SERVE_RESIDENT  a persistent kernel stays on the device while there
- *    is work: submit publishes a descriptor in pinned memory, one wave of
- *    the kernel follows the descriptor ring, workgroups claim events one by
- *    one, read the event from the pinned input arena, search it and write
- *    its tracks straight into the request's pinned output region, then
- *    append a completion record.  A host thread collects completions and
+ *    is work.  Submit copies the event into pinned staging; an uploader
+ *    thread moves the newly submitted events into a device input arena
+ *    with the copy engine and then publishes their descriptors in pinned
+ *    memory.  One wave of the kernel follows the descriptor ring;
+ *    workgroups claim events one by one, copy the event from the device
+ *    arena, search it, write its tracks into a device output ring and
+ *    append a completion record.  A host thread copies the finished part of
+ *    the output ring back with the copy engine, hands out the results, and
  *    relaunches the kernel after it has left for lack of work (idle_us).
+ *    Admission is by device memory: the input arena, and room in the output
+ *    ring for every admitted event's capacity bound of tracks.
  *  CLF_SERVE_BATCHES   a dispatcher thread keeps `slots` batches in flight,
  *    each on its own stream with resident device buffers, and forms the
  *    next batch from everything that arrived meanwhile; tracks come back
  *    when the whole batch has finished.
  *
  * Threading: clf_server_submit, _wait and _release may be called from any
- * number of threads.  Admission: a submit that would exceed a pinned arena,
- * the request table or the engine's per-event or per-batch limits returns
+ * number of threads.  Admission: a submit that would exceed an arena, the
+ * request table or the engine's per-event or per-batch limits returns
  * CLF_E_CAPACITY at once (nothing is queued). */
 typedef struct clf_server clf_server;
 
@@ -242,8 +247,10 @@ typedef enum clf_serve_engine { CLF_SERVE_RESIDENT = 0, CLF_SERVE_BATCHES = 1 } 
 typedef struct clf_server_config {
     int32_t engine;           /* clf_serve_engine                               */
     int32_t max_requests;     /* submitted and not yet released                 */
-    int64_t arena_bytes;      /* pinned input arena                             */
-    int64_t out_arena_bytes;  /* resident: pinned output arena                  */
+    int64_t arena_bytes;      /* input arena: pinned staging (and, resident, the
+                                 device input arena of the same size)           */
+    int64_t out_arena_bytes;  /* resident: device output ring (and its pinned
+                                 mirror)                                        */
     int32_t max_event_hits;   /* resident: largest event (workgroup scratch)    */
     int32_t idle_us;          /* resident: the kernel leaves after this long
                                  without work                                   */
@@ -253,11 +260,6 @@ typedef struct clf_server_config {
     int32_t max_wait_us;      /* batches: while another batch runs, a batch
                                  waits up to this long to fill; an idle device
                                  launches at once                               */
-    int32_t out_tracks_per_1k_hits; /* resident: output region of a request,
-                                 out_min_tracks + N * this / 1000 records (at
-                                 most the capacity bound); an event with more
-                                 tracks is searched again on a side stream     */
-    int32_t out_min_tracks;
     int32_t reserved;
 } clf_server_config;
 
@@ -272,14 +274,20 @@ typedef struct clf_served {
     int64_t n_tracks;
     const uint8_t* tracks;    /* n_tracks * 100 bytes                           */
     int64_t t_submit_ns;      /* steady clock: submit returned                  */
-    int64_t t_launch_ns;      /* batches: its batch was enqueued; resident: 0   */
+    int64_t t_launch_ns;      /* batches: its batch was enqueued; resident: its
+                                 event was in device memory and its descriptor
+                                 published                                      */
     int64_t t_done_ns;        /* its tracks were visible on the host            */
+    int64_t t_complete_ns;    /* its search had finished: resident, its
+                                 completion record was seen on the host;
+                                 batches, its batch was seen finished           */
 } clf_served;
 
 /* Fills *cfg with the defaults: resident engine, 65536 requests, 512 MiB
- * input and 1 GiB output arenas, events up to 32768 hits, 5000 us idle,
- * output regions of 256 + N/2 records; for the batch engine 4 slots of 256
- * events / 2^20 hits, 100 us. */
+ * input arena and 1 GiB output ring, events up to 32768 hits, 5000 us idle;
+ * for the batch engine 4 slots of 256 events / 2^20 hits, 100 us.  The
+ * resident engine's device arenas are fine-grained device memory, which
+ * the library keeps for reuse by later servers until the process ends. */
 void clf_server_default_config(clf_server_config* cfg);
 /* Starts a server on the context's device with the context's launch
  * configuration.  Allocates every buffer up front. */
@@ -322,13 +330,16 @@ clf_status clf_server_submit_framed(clf_server* server, const uint8_t* frame, si
  * counted from the scheduled arrival to the result visible on the host. */
 typedef struct clf_replay_record {
     int64_t latency_ns;       /* -1 if rejected                                 */
-    int64_t wait_ns;          /* batches: scheduled arrival to batch launch;
-                                 resident: -1                                   */
+    int64_t wait_ns;          /* scheduled arrival to dispatch: batches, its
+                                 batch's launch; resident, its descriptor's
+                                 publication (its event in device memory)       */
     uint64_t digest;
     int32_t n_tracks;
     int32_t status;           /* CLF_OK, or why the submit or the run failed    */
     int32_t batch_events;
     uint32_t flags;
+    int64_t device_ns;        /* dispatch to the search seen finished (the rest
+                                 of the latency is waiting and copying)         */
 } clf_replay_record;
 
 typedef struct clf_replay_summary {
@@ -336,11 +347,14 @@ typedef struct clf_replay_summary {
     double events_per_s;      /* completed / wall_s                             */
     int64_t completed, rejected;
     int64_t p50_ns, p90_ns, p99_ns, max_ns;
-    int64_t batches;
-    double mean_batch_events;
+    int64_t batches;          /* batches: batches launched; resident: launches
+                                 of the persistent kernel                       */
+    double mean_batch_events; /* batches: events per batch; resident: 1 (each
+                                 event is dispatched alone)                     */
     double max_submit_lag_ns; /* how late the submitters ran at worst           */
     /* resident engine, device wall clock per event (means, microseconds):
-     * input copy over the host link, search, track copy and completion */
+     * event copy from the device arena into the workgroup's scratch, search,
+     * track copy into the output ring and completion */
     double dev_copy_in_us, dev_search_us, dev_copy_out_us;
     double dev_busy_frac;     /* workgroup time with an event / (workers x wall) */
     double host_submit_us;    /* mean submit time per arrival (a late submitter
