@@ -1302,7 +1302,7 @@ void clf_server_default_config(clf_server_config* cfg) {
     cfg->engine = CLF_SERVE_RESIDENT;
     cfg->max_requests = 65536;
     cfg->arena_bytes = (int64_t)512 << 20;
-    cfg->out_arena_bytes = (int64_t)1 << 30;  // device output ring
+    cfg->out_arena_bytes = (int64_t)2 << 30;  // device output ring
     cfg->max_event_hits = 32768;
     cfg->idle_us = 5000;
     cfg->slots = 4;

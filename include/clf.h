@@ -284,7 +284,7 @@ typedef struct clf_served {
 } clf_served;
 
 /* Fills *cfg with the defaults: resident engine, 65536 requests, 512 MiB
- * input arena and 1 GiB output ring, events up to 32768 hits, 5000 us idle;
+ * input arena and 2 GiB output ring, events up to 32768 hits, 5000 us idle;
  * for the batch engine 4 slots of 256 events / 2^20 hits, 100 us.  The
  * resident engine's device arenas are fine-grained device memory, which
  * the library keeps for reuse by later servers until the process ends. */
