@@ -66,7 +66,7 @@ class ServerConfig(ctypes.Structure):
         ("max_batch_events", ctypes.c_int32),
         ("max_batch_hits", ctypes.c_int64),
         ("max_wait_us", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("feeder_pause_us", ctypes.c_int32),
     ]
 
 

@@ -121,14 +121,16 @@ struct CompRec {
 // Control words in device memory.  HEAD (descriptors handed to workers),
 // TAIL (descriptors seen published), DONE (completions written) and the
 // output ring's allocator OUT_HEAD (64-bit, words 8-9) persist across
-// launches; QUIT, IDLE_TAIL and the feeder's heartbeat BEAT (64-bit, words
-// 6-7) are cleared before each launch.
+// launches; QUIT, IDLE_TAIL, LEFT (workers that have left) and the
+// feeder's heartbeat BEAT (64-bit, words 6-7) are cleared before each
+// launch.
 enum ServeCtl {
     CTL_HEAD = 0,
     CTL_TAIL = 1,
     CTL_QUIT = 2,
     CTL_DONE = 3,
     CTL_IDLE_TAIL = 4,
+    CTL_LEFT = 5,
     CTL_BEAT = 6,
     CTL_OUT_HEAD = 8,
     CTL_WORDS = 16
@@ -152,6 +154,8 @@ struct ServeArgs {
     int64_t wg_hits;
     int64_t wg_cap;
     uint64_t idle_ticks;       // wall-clock ticks without work before leaving
+    uint64_t pause_ticks;      // testing: the feeder stalls this long, without a
+                               // heartbeat, before its first hand-out (0: never)
 };
 
 }  // namespace clf

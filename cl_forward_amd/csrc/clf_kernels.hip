@@ -1765,15 +1765,20 @@ void search_kernel(SearchArgs a) {
 // appends a completion record to a ring in pinned memory; the host copies
 // the finished part of the output ring back with the copy engine.
 //
-// Termination: the feeder leaves when the host sets its stop word, or when
+// Termination: the feeder leaves when the host sets its stop word, when
 // every published descriptor has been handed out and nothing has moved for
-// idle_ticks; it then sets CTL_QUIT.  A waiting worker leaves on CTL_QUIT
-// (after one more look at its mailbox: the feeder writes every hand-out
-// before CTL_QUIT), or when the feeder's heartbeat has been silent for
-// 4 x idle_ticks: then it withdraws its idle entry with a compare-and-swap
-// first, and if the feeder has claimed the entry already (the swap fails),
-// it stays for the hand-out, which the feeder writes right after the claim.
-// Every wait is bounded once the host stops publishing.
+// idle_ticks, or when every worker has left (CTL_LEFT); it then stores
+// CTL_HEAD, from which the next launch resumes, and sets CTL_QUIT.  A
+// waiting worker leaves on CTL_QUIT (after one more look at its mailbox: the
+// feeder writes every hand-out before CTL_QUIT), or when the feeder's
+// heartbeat has been silent for 4 x idle_ticks (a feeder wave held off the
+// chip): then it withdraws its idle entry with a compare-and-swap first, and
+// if the feeder has claimed the entry already (the swap fails), it stays for
+// the hand-out, which the feeder writes right after the claim.  A feeder
+// that comes back to withdrawn entries passes them over; once no worker is
+// left it leaves with descriptors still waiting, and the host, which sees
+// the kernel end with work outstanding, launches it again.  Every wait is
+// bounded once the host stops publishing.
 
 // Polls are relaxed (no cache invalidation per poll); the one acquire per
 // event is the worker's check of its descriptor's sequence word.
@@ -1805,6 +1810,8 @@ __device__ void serve_feeder(const ServeArgs& a) {
     uint32_t h = poll_agent(&ctl[CTL_HEAD]);
     uint32_t ih = 0;  // idle ring read position (the ring is cleared per launch)
     uint64_t last = wall_clock64();
+    uint64_t pause = a.pause_ticks;
+    const uint32_t n_workers = gridDim.x - 1u;
     bool wide = false;
     for (;;) {
         const uint64_t now = wall_clock64();
@@ -1822,6 +1829,13 @@ __device__ void serve_feeder(const ServeArgs& a) {
             }
         }
         if (t != h) {
+            if (pause != 0) {
+                // testing: one bounded stall with no heartbeat before the
+                // first hand-out, as if this wave had been held off the chip
+                const uint64_t p0 = wall_clock64();
+                while (wall_clock64() - p0 < pause) __builtin_amdgcn_s_sleep(16);
+                pause = 0;
+            }
             // the written prefix of the idle ring from ih, at most one entry
             // per waiting descriptor; each lane claims its entry, and the
             // claimed ones take the descriptors in lane order (an entry its
@@ -1854,6 +1868,7 @@ __device__ void serve_feeder(const ServeArgs& a) {
         }
         if (poll_sys(a.host_stop) != 0u) break;
         if (h == t && now - last > a.idle_ticks) break;
+        if (poll_agent(&ctl[CTL_LEFT]) >= n_workers) break;
         __builtin_amdgcn_s_sleep(2);
     }
     if (lane == 0) {
@@ -2046,6 +2061,8 @@ void serve_kernel(ServeArgs a) {
             }
             sh.item = got;
             sh.sv_d = d;
+            if (got < 0)  // leaving: counted so that a feeder left alone leaves too
+                __hip_atomic_fetch_add(&ctl[CTL_LEFT], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();
         const int got = uni(sh.item);

@@ -349,6 +349,8 @@ struct clf_server {
     hipStream_t rstream = nullptr;
     hipEvent_t ev_kernel = nullptr;
     bool kernel_running = false;
+    int64_t launches = 0;  // resident: kernel launches (the resident thread's own count)
+    int rate_khz = 0;      // resident: device wall clock
     // device phase sums over collected completions (wall-clock ticks)
     uint64_t dev_in = 0, dev_search = 0, dev_out = 0, dev_n = 0;
     double tick_ns = 10.0;
@@ -710,6 +712,8 @@ ServeArgs serve_args(clf_server* s) {
     if (rate_khz <= 0) rate_khz = 100000;
     s->tick_ns = 1.0e6 / (double)rate_khz;
     a.idle_ticks = (uint64_t)s->conf.idle_us * (uint64_t)rate_khz / 1000ull;
+    a.pause_ticks = 0;
+    s->rate_khz = rate_khz;
     return a;
 }
 
@@ -719,7 +723,9 @@ clf_status launch_resident(clf_server* s) {
     HIP_TRY(launch_zero_i32(s->ctl.as<int32_t>() + CTL_IDLE_TAIL, CTL_OUT_HEAD - CTL_IDLE_TAIL, s->rstream));
     HIP_TRY(launch_zero_i32(s->mailbox.as<int32_t>(), s->grid, s->rstream));
     HIP_TRY(launch_zero_i32(s->idle_ring.as<int32_t>(), 2 * (int)(s->idle_mask + 1u), s->rstream));
-    const ServeArgs a = serve_args(s);
+    ServeArgs a = serve_args(s);
+    if (s->launches++ == 0)  // the testing stall, first launch only
+        a.pause_ticks = (uint64_t)s->conf.feeder_pause_us * (uint64_t)s->rate_khz / 1000ull;
     HIP_TRY(launch_serve(a, s->cfg.block_size, s->cfg.lds_hits, s->grid, s->rstream));
     HIP_TRY(hipEventRecord(s->ev_kernel, s->rstream));
     s->kernel_running = true;
@@ -1309,7 +1315,7 @@ void clf_server_default_config(clf_server_config* cfg) {
     cfg->max_batch_events = 256;
     cfg->max_batch_hits = 1 << 20;
     cfg->max_wait_us = 100;
-    cfg->reserved = 0;
+    cfg->feeder_pause_us = 0;
 }
 
 void clf_server_destroy(clf_server* s) {
@@ -1370,7 +1376,8 @@ clf_status clf_server_create(clf_context* ctx, const clf_server_config* cfg, clf
         return fail(CLF_E_INVALID_ARGUMENT, "bad server configuration");
     if (c.engine == CLF_SERVE_RESIDENT &&
         (c.max_event_hits < 1 || c.max_event_hits > CLF_MAX_SENSOR_HITS * 64 || c.idle_us < 1 ||
-         c.idle_us > 10000000 || c.out_arena_bytes < (int64_t)(1 << 20)))
+         c.idle_us > 10000000 || c.out_arena_bytes < (int64_t)(1 << 20) || c.feeder_pause_us < 0 ||
+         c.feeder_pause_us > 10000000))
         return fail(CLF_E_INVALID_ARGUMENT, "bad resident server configuration");
     if (c.engine == CLF_SERVE_BATCHES &&
         (c.slots < 1 || c.slots > kMaxSlots || c.max_batch_events < 1 ||

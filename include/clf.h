@@ -260,7 +260,10 @@ typedef struct clf_server_config {
     int32_t max_wait_us;      /* batches: while another batch runs, a batch
                                  waits up to this long to fill; an idle device
                                  launches at once                               */
-    int32_t reserved;
+    int32_t feeder_pause_us;  /* resident, testing: in the first launch the
+                                 kernel's feeder stalls this long, with no
+                                 heartbeat, before its first hand-out (0 in
+                                 use)                                           */
 } clf_server_config;
 
 /* A finished request.  `tracks` points into the server's pinned output and

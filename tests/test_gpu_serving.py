@@ -179,6 +179,35 @@ def test_resident_idle_exit_and_relaunch(gpu_ctx, plain51_blobs, plain_ref):
             time.sleep(0.05)  # 25 x idle_us: the kernel has left
 
 
+def test_resident_feeder_stall_loses_no_request(gpu_ctx, plain51_blobs, plain_ref):
+    """The first launch's feeder stalls 10 x idle_us with no heartbeat
+    before its first hand-out (feeder_pause_us, a testing control; the
+    stall is bounded, so every wait still ends).  Each waiting worker
+    withdraws its idle entry after 4 x idle_us and leaves; the feeder passes
+    the withdrawn entries over and, with no worker left, leaves with the
+    requests still waiting; the host launches the kernel again, and every
+    request is served exactly.  A hand-out to a departed worker would lose
+    its request (its wait times out), and a feeder that stayed on alone
+    would hold every request until the server closed."""
+    ref_outs, _ = plain_ref
+    idle_us, pause_us = 2000, 20000
+    picks = [i % len(plain51_blobs) for i in range(64)]
+    with Server(gpu_ctx, engine=SERVE_RESIDENT, idle_us=idle_us, feeder_pause_us=pause_us) as srv:
+        tickets = srv.submit_many([plain51_blobs[i] for i in picks])
+        assert len(tickets) == len(picks)
+        lat_ms = []
+        for t, i in zip(tickets, picks):
+            raw, _, tm = srv.wait(t, timeout_ms=20000)
+            assert raw == ref_outs[i], i
+            lat_ms.append((tm["done_ns"] - tm["submit_ns"]) / 1e6)
+        # the stall happened: nothing came back before it ended
+        assert min(lat_ms) > 0.75 * pause_us / 1000.0, lat_ms[:4]
+        # later requests on the same server see no stall
+        for i in (3, 11):
+            raw, _, tm = srv.wait(srv.submit(plain51_blobs[i]), timeout_ms=20000)
+            assert raw == ref_outs[i]
+
+
 def test_resident_output_ring_wraps_and_keeps_unreleased(gpu_ctx, plain51_blobs, plain_ref):
     """A 4 MiB device output ring, where each admitted event reserves its
     capacity bound (about 0.6 MB) and uses about 28 kB: after one round of
