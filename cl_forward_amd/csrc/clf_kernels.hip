@@ -1140,6 +1140,18 @@ __device__ __forceinline__ int n_w1_items(const Shared<B, C>& sh, int g1) {
     return g1 >= 4 ? uni(sh.s_num[g1]) : 0;
 }
 
+// Track records in an event's slice hold hitsNum and hits[0, hitsNum)
+// only; the zero padding of the 100-byte output record (Kernel
+// Definitions.h:59-62) is written when the record leaves the slice, so the
+// slice is neither written nor read past a track's last hit.  Word k of the
+// output records whose slice copy starts at src:
+constexpr uint32_t kTrackWords = (uint32_t)(sizeof(clf_track) / 4);
+__device__ __forceinline__ uint32_t track_word(const uint32_t* src, uint32_t k) {
+    const uint32_t r = k / kTrackWords;
+    const uint32_t n = src[r * kTrackWords];  // hitsNum
+    return k - r * kTrackWords <= n ? src[k] : 0u;
+}
+
 // Sensor-loop state of one event (Kernel.cl:669-767).
 struct StepState {
     int n_cur, n_tracks, n_tracklets, n_weak, visits;
@@ -1290,8 +1302,6 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
                     r[2] = t.t1;
                     r[3] = t.t2;
                     r[4] = (uint32_t)b;
-#pragma unroll
-                    for (int q = 5; q < 1 + CLF_MAX_TRACK_SIZE; ++q) r[q] = 0;
                 } else {
                     slot = t.word & CLF_TTF_INDEX_MASK;
                     tracks[slot].hits[t.nhits] = (uint32_t)b;
@@ -1652,8 +1662,6 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
             r[1] = tl.x;
             r[2] = tl.y;
             r[3] = tl.z;
-#pragma unroll
-            for (int q = 4; q < 1 + CLF_MAX_TRACK_SIZE; ++q) r[q] = 0;
         }
         n_tracks += tot[0];
     }
@@ -1690,22 +1698,12 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
         const int64_t base = uni(sh.pack_base);
         const int nt = uni(sh.item);
         const int64_t toff = uni(a.ev[e].track_off);
-        // the source slice starts 16-byte aligned (slices are multiples of 4
-        // records); a destination record keeps 4-byte alignment only: 16-byte
-        // loads, 4-byte stores, then the words past the last 16-byte one
-        const uint4* src = reinterpret_cast<const uint4*>(a.tracks + toff);
+        // consecutive lanes write consecutive words; the zero padding is
+        // made here (track_word)
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(a.tracks + toff);
         uint32_t* dst = reinterpret_cast<uint32_t*>(a.packed + base);
-        const int n16 = (nt * (int)sizeof(clf_track)) / 16;
-        for (int k = tid; k < n16; k += B) {
-            const uint4 v = src[k];
-            dst[4 * k] = v.x;
-            dst[4 * k + 1] = v.y;
-            dst[4 * k + 2] = v.z;
-            dst[4 * k + 3] = v.w;
-        }
-        const int words = nt * (int)(sizeof(clf_track) / 4);
-        const uint32_t* src4 = reinterpret_cast<const uint32_t*>(a.tracks + toff);
-        for (int k = 4 * n16 + tid; k < words; k += B) dst[k] = src4[k];
+        const uint32_t words = (uint32_t)nt * kTrackWords;
+        for (uint32_t k = tid; k < words; k += B) dst[k] = track_word(src, k);
     }
     prof_mark(sh, pc, PROF_TAIL);
     if (pc.on && tid == 0) {
@@ -1962,8 +1960,8 @@ __device__ void serve_one(const ServeArgs& a, int w, Shared<B, C>& sh, uint64_t 
         const int64_t ring_words = (int64_t)(a.out_bytes / 4);
         const int64_t w0 = (int64_t)((pos / 4) % (uint64_t)ring_words);
         const int64_t first = min(words, ring_words - w0);  // words before the ring's end
-        for (int64_t k = tid; k < first; k += B) ring[w0 + k] = ts[k];
-        for (int64_t k = first + tid; k < words; k += B) ring[k - first] = ts[k];
+        for (int64_t k = tid; k < first; k += B) ring[w0 + k] = track_word(ts, (uint32_t)k);
+        for (int64_t k = first + tid; k < words; k += B) ring[k - first] = track_word(ts, (uint32_t)k);
     }
     // every wave's ring stores are complete; thread 0's system-scope release
     // of the completion below then makes them visible to the copy engine
