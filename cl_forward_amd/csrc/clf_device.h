@@ -89,8 +89,29 @@ constexpr int kProfSlots = 16;
 // A request published by the host in pinned memory once its event blob is
 // in the device input arena.  `seq` is written last (release): descriptor d
 // of the ring is valid once seq == d + 1.
+// The part of an event blob the search reads, as the serving input arena
+// holds it: the blob's 16-byte words [0, h16) (header and sensor table),
+// then its words [x16, n16) (from the one holding X[0]: the hit IDs between
+// are never read), then its last bytes - bytes - 16 n16 - past word n16.
+// Word j < m16 of the packed form is blob word j < h16 ? j : j + x16 - h16.
+struct SearchedWords {
+    int64_t bytes, h16, x16, n16, m16, packed;
+};
+constexpr SearchedWords searched_words(int32_t S, int32_t N) {
+    SearchedWords w{};
+    w.bytes = 8 + 12 * (int64_t)S + 16 * (int64_t)N;
+    w.n16 = w.bytes >> 4;
+    w.h16 = (8 + 12 * (int64_t)S + 15) >> 4;
+    const int64_t x = (8 + 12 * (int64_t)S + 4 * (int64_t)N) >> 4;
+    w.x16 = x > w.h16 ? x : w.h16;
+    w.m16 = w.h16 + (w.n16 - w.x16);
+    w.packed = 16 * w.m16 + (w.bytes - 16 * w.n16);
+    return w;
+}
+
 struct alignas(16) ServeDesc {
-    uint64_t in_off;   // event blob in the device input arena (256-byte aligned)
+    uint64_t in_off;   // event, in its searched-words form (SearchedWords), in
+                       // the device input arena (256-byte aligned)
     uint64_t reserved;
     int32_t n_hits;
     int32_t n_sensors;

@@ -1881,14 +1881,15 @@ __device__ void serve_one(const ServeArgs& a, int w, Shared<B, C>& sh, uint64_t 
     const int tid = threadIdx.x;
     const uint64_t in_off = sh.sv_in;
     const int N = sh.sv_n, S = sh.sv_s, req = sh.sv_req, out_cap = sh.sv_cap;
-    const int64_t bytes = 8 + 12 * (int64_t)S + 16 * (int64_t)N;
+    const SearchedWords sw = searched_words(S, N);
+    const int64_t bytes = sw.bytes;
     // the host checked these at submit; a descriptor outside them is
     // answered with CLF_FLAG_INVALID instead of being read
     const int64_t cap = CLF_EVENT_TRACK_CAPACITY(N, S);
     const bool fits = N >= 0 && S >= 0 && S <= CLF_MAX_SENSORS && N <= a.wg_hits &&
                       bytes <= a.wg_blob_bytes && cap <= a.wg_cap && out_cap >= 0 &&
                       (int64_t)out_cap * (int64_t)sizeof(clf_track) <= (int64_t)a.out_bytes &&
-                      in_off + (uint64_t)bytes <= a.in_bytes && (in_off & 15u) == 0;
+                      in_off + (uint64_t)sw.packed <= a.in_bytes && (in_off & 15u) == 0;
     // the workgroup's own scratch (SearchArgs holds it as read-only input)
     uint8_t* blob = const_cast<uint8_t*>(a.base.blobs) + (size_t)w * (size_t)a.wg_blob_bytes;
     if (fits) {
@@ -1898,30 +1899,26 @@ __device__ void serve_one(const ServeArgs& a, int w, Shared<B, C>& sh, uint64_t 
         // barrier every wave passed since
         //
         // copy-in from the device arena, four 16-byte loads in flight per
-        // lane; the search never reads the hit IDs (N words after the sensor
-        // table), so only the words of the header and sensor table [0, h16)
-        // and those from the one holding X[0] on, [x16, n16), are copied
+        // lane: the arena holds the event's searched words packed (the host
+        // left out the hit IDs, which the search never reads), and they go
+        // to their places in the blob layout the search reads
         const uint4* src = reinterpret_cast<const uint4*>(a.in_arena + in_off);
         uint4* dst = reinterpret_cast<uint4*>(blob);
-        const int64_t n16 = bytes >> 4;
-        const int64_t h16 = (8 + 12 * (int64_t)S + 15) >> 4;
-        const int64_t x16 = max(h16, (8 + 12 * (int64_t)S + 4 * (int64_t)N) >> 4);
-        const int64_t m16 = h16 + (n16 - x16);  // words copied; word j < h16 is j, else j - h16 + x16
-        const int64_t gap = x16 - h16;
+        const int64_t h16 = sw.h16, m16 = sw.m16, gap = sw.x16 - sw.h16;
         auto at = [&](int64_t j) { return j < h16 ? j : j + gap; };
         int64_t i = tid;
         for (; i + 3 * B < m16; i += 4 * B) {
-            const int64_t j0 = at(i), j1 = at(i + B), j2 = at(i + 2 * B), j3 = at(i + 3 * B);
-            const uint4 v0 = src[j0], v1 = src[j1], v2 = src[j2], v3 = src[j3];
-            dst[j0] = v0;
-            dst[j1] = v1;
-            dst[j2] = v2;
-            dst[j3] = v3;
+            const uint4 v0 = src[i], v1 = src[i + B], v2 = src[i + 2 * B], v3 = src[i + 3 * B];
+            dst[at(i)] = v0;
+            dst[at(i + B)] = v1;
+            dst[at(i + 2 * B)] = v2;
+            dst[at(i + 3 * B)] = v3;
         }
-        for (; i < m16; i += B) dst[at(i)] = src[at(i)];
-        const uint32_t* s4 = reinterpret_cast<const uint32_t*>(a.in_arena + in_off);
-        uint32_t* d4 = reinterpret_cast<uint32_t*>(blob);
-        for (int64_t k = n16 * 4 + tid; k < bytes / 4; k += B) d4[k] = s4[k];
+        for (; i < m16; i += B) dst[at(i)] = src[i];
+        // the blob's last bytes past word n16, after the packed words
+        const uint32_t* s4 = reinterpret_cast<const uint32_t*>(a.in_arena + in_off) + 4 * m16;
+        uint32_t* d4 = reinterpret_cast<uint32_t*>(blob) + 4 * sw.n16;
+        for (int64_t k = tid; k < (bytes - 16 * sw.n16) / 4; k += B) d4[k] = s4[k];
     }
     if (tid == 0) {
         EventDesc ed;

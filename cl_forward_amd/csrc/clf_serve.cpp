@@ -1171,6 +1171,15 @@ clf_status create_batches(clf_server* s) {
     return CLF_OK;
 }
 
+// Writes the words of an event blob that the search reads, packed
+// (SearchedWords), to dst: what the resident engine stages and uploads.
+void stage_searched(uint8_t* dst, const uint8_t* blob, const SearchedWords& w) {
+    std::memcpy(dst, blob, 16 * (size_t)std::min(w.h16, w.m16));
+    if (w.m16 > w.h16)
+        std::memcpy(dst + 16 * w.h16, blob + 16 * w.x16, 16 * (size_t)(w.m16 - w.h16));
+    std::memcpy(dst + 16 * w.m16, blob + 16 * w.n16, (size_t)(w.bytes - 16 * w.n16));
+}
+
 // Submits events[0..n) in order and stops at the first one that is
 // malformed or does not fit: `*accepted` of them are queued, and the status
 // is that of the first one refused (CLF_OK if none was).  The requests, arena
@@ -1181,7 +1190,7 @@ clf_status submit_events(clf_server* s, int32_t n, const uint8_t* const* events,
                          const uint64_t* tags, uint64_t* tickets, int32_t* accepted) {
     struct Sized {
         int32_t S, N;
-        size_t bytes, span;
+        size_t bytes, staged, span;  // blob, staged form, its arena span
         int64_t cap;
         uint64_t out_cap_bytes;
     };
@@ -1198,11 +1207,14 @@ clf_status submit_events(clf_server* s, int32_t n, const uint8_t* const* events,
             break;
         }
         e.bytes = event_bytes(e.S, e.N);
-        e.span = align_up(e.bytes, kBlobAlign);
+        // the resident engine ships only the words the search reads
+        e.staged = s->resident ? (size_t)searched_words(e.S, e.N).packed : e.bytes;
+        e.span = align_up(e.staged, kBlobAlign);
         e.cap = track_slice(e.N, e.S);
         e.out_cap_bytes = (uint64_t)CLF_EVENT_TRACK_CAPACITY(e.N, e.S) * sizeof(clf_track);
         if (s->resident) {
-            if (e.N > s->wg_hits || e.cap > s->wg_cap || (int64_t)e.span > s->wg_blob_bytes ||
+            if (e.N > s->wg_hits || e.cap > s->wg_cap ||
+                (int64_t)align_up(e.bytes, kBlobAlign) > s->wg_blob_bytes ||
                 e.out_cap_bytes > s->out_ring) {
                 refused = CLF_E_CAPACITY, why = "event larger than the serving kernel's scratch";
                 break;
@@ -1285,7 +1297,10 @@ clf_status submit_events(clf_server* s, int32_t n, const uint8_t* const* events,
     for (int32_t k = 0; k < taken_n; ++k) {
         Request* r = rq[k];
         const Sized& e = z[k];
-        std::memcpy(s->arena.as<uint8_t>() + r->in_off, events[k], e.bytes);
+        if (s->resident)
+            stage_searched(s->arena.as<uint8_t>() + r->in_off, events[k], searched_words(e.S, e.N));
+        else
+            std::memcpy(s->arena.as<uint8_t>() + r->in_off, events[k], e.bytes);
         r->cost = event_cost(events[k], e.S, e.N);
         r->out.t_submit_ns = now_ns();
         // a failure of the server may have ended the request meanwhile; it
