@@ -5,8 +5,9 @@
 // engines take it from there:
 //
 // Resident: a persistent kernel (serve_kernel) follows a descriptor ring in
-// pinned memory.  Submit copies the event into pinned staging; the server's
-// thread moves each run of newly submitted events into the device input
+// pinned memory.  Submit copies the words of the event that the search
+// reads (all but the hit IDs) into pinned staging; the server's thread
+// moves each run of newly submitted events into the device input
 // arena with one copy-engine transfer and, once it has landed, publishes
 // their descriptors (release store of each sequence word).  A workgroup
 // claims a descriptor, copies the event from the device arena into its
