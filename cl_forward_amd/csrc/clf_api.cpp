@@ -146,6 +146,21 @@ int resolved_grid(const clf_launch_config& cfg, int n_cu, int n_items) {
     return std::max(1, std::min(grid, n_items));
 }
 
+clf_launch_config resolve_launch(const clf_launch_config& cfg, int64_t hits, int32_t n) {
+    clf_launch_config r = cfg;
+    if (r.block_size == 0) {
+        // the tuned launches of the two workload shapes: 2.8k-hit events
+        // (batch1000: 0.86 ms at 512 x 256 against 1.39 ms at 1024 x 512) and
+        // 7.1k-hit events (stress1000: 2.67 ms at 1024 x 512 against 5.66 ms)
+        const bool large = n > 0 && hits > kAutoLargeEventHits * (int64_t)n;
+        r.block_size = large ? 1024 : 512;
+        r.lds_hits = large ? 512 : 256;
+    } else if (r.lds_hits == 0) {
+        r.lds_hits = 256;
+    }
+    return r;
+}
+
 clf_status enqueue_search(const Resident& r, int32_t n, clf_mode mode,
                           const clf_launch_config& cfg, int n_cu, hipStream_t st, hipEvent_t t0,
                           hipEvent_t t1, hipStream_t side, hipEvent_t fork, hipEvent_t join) {
@@ -199,7 +214,7 @@ struct clf_context {
     char name[256] = {0};
     hipStream_t stream = nullptr;
     int n_cu = 0;
-    clf_launch_config cfg = {256, 0, 1, 1, 256, 0, 512};
+    clf_launch_config cfg = {0, 0, 1, 1, 0, 0, 512};  // automatic launch shape
     hipStream_t side = nullptr;  // the heavy-event launch of the batched mode
     hipEvent_t fork = nullptr, join = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
@@ -243,7 +258,8 @@ hipError_t current_device(clf_context* ctx) { return hipSetDevice(ctx->device); 
 
 // The kernels of one run, enqueued on the context stream.
 clf_status enqueue_run(clf_context* ctx, clf_batch* b, clf_mode mode, bool time_search) {
-    return enqueue_search(b->dev, b->n, mode, ctx->cfg, ctx->n_cu, ctx->stream,
+    return enqueue_search(b->dev, b->n, mode, resolve_launch(ctx->cfg, b->total_hits, b->n),
+                          ctx->n_cu, ctx->stream,
                           time_search ? ctx->ev2 : nullptr, time_search ? ctx->ev3 : nullptr,
                           ctx->side, ctx->fork, ctx->join);
 }
@@ -258,7 +274,8 @@ clf_status run(clf_context* ctx, clf_batch* b, clf_mode mode) {
     HIP_TRY(current_device(ctx));
     b->results_valid = false;
     if (mode == CLF_MODE_BATCHED && ctx->cfg.use_graph && b->n > 0) {
-        if (b->graph && !same_cfg(b->graph_cfg, ctx->cfg)) {
+        const clf_launch_config lc = resolve_launch(ctx->cfg, b->total_hits, b->n);
+        if (b->graph && !same_cfg(b->graph_cfg, lc)) {
             (void)hipGraphExecDestroy(b->graph);
             b->graph = nullptr;
         }
@@ -275,7 +292,7 @@ clf_status run(clf_context* ctx, clf_batch* b, clf_mode mode) {
             e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
-            b->graph_cfg = ctx->cfg;
+            b->graph_cfg = lc;
         }
         HIP_TRY(hipGraphLaunch(b->graph, ctx->stream));
         return CLF_OK;
@@ -377,13 +394,18 @@ void clf_context_destroy(clf_context* ctx) {
 
 clf_status clf_context_set_config(clf_context* ctx, const clf_launch_config* cfg) {
     if (!ctx || !cfg) return fail(CLF_E_INVALID_ARGUMENT, "null argument");
-    if (!config_supported(cfg->block_size, cfg->lds_hits))
+    if (cfg->block_size != 0 &&
+        !config_supported(cfg->block_size, cfg->lds_hits == 0 ? 256 : cfg->lds_hits))
         return fail(CLF_E_INVALID_ARGUMENT,
-                    "unsupported (block_size, lds_hits): block 64/128/256/512/1024, lds_hits 256/512");
+                    "unsupported (block_size, lds_hits): block 0 (automatic) or 64/128/256/512/1024, "
+                    "lds_hits 0/256/512");
+    if (cfg->lds_hits != 0 && cfg->lds_hits != 256 && cfg->lds_hits != 512)
+        return fail(CLF_E_INVALID_ARGUMENT, "lds_hits must be 0, 256 or 512");
     if (cfg->grid_size < 0) return fail(CLF_E_INVALID_ARGUMENT, "grid_size must be >= 0");
     if (cfg->grain < 1 || cfg->grain > 64) return fail(CLF_E_INVALID_ARGUMENT, "grain must be 1..64");
     if (cfg->heavy_events < 0 ||
-        (cfg->heavy_events > 0 && !config_supported(cfg->heavy_block, cfg->lds_hits)))
+        (cfg->heavy_events > 0 && !(config_supported(cfg->heavy_block, 256) &&
+                                    config_supported(cfg->heavy_block, 512))))
         return fail(CLF_E_INVALID_ARGUMENT, "heavy_events must be >= 0 with a supported heavy_block");
     ctx->cfg = *cfg;
     return CLF_OK;
@@ -684,16 +706,17 @@ clf_status clf_batch_profile(clf_context* ctx, clf_batch* b, clf_mode mode, uint
     hipStream_t st = ctx->stream;
     SearchArgs a = b->dev.args(n);
     a.prof = prof.as<unsigned long long>();
+    const clf_launch_config lc = resolve_launch(ctx->cfg, b->total_hits, n);
     HIP_TRY(launch_zero_i32(b->dev.queue.as<int32_t>(), n + 1, st, a.track_alloc));
     if (mode == CLF_MODE_BATCHED) {
-        HIP_TRY(launch_search(a, ctx->cfg.block_size, ctx->cfg.lds_hits, resolved_grid(ctx->cfg, ctx->n_cu, n), st));
+        HIP_TRY(launch_search(a, lc.block_size, lc.lds_hits, resolved_grid(lc, ctx->n_cu, n), st));
     } else {
         a.order = nullptr;
         a.n_items = 1;
         for (int i = 0; i < n; ++i) {
             a.first_item = i;
             a.queue_head = b->dev.queue.as<int32_t>() + 1 + i;
-            HIP_TRY(launch_search(a, ctx->cfg.block_size, ctx->cfg.lds_hits, 1, st));
+            HIP_TRY(launch_search(a, lc.block_size, lc.lds_hits, 1, st));
         }
     }
     HIP_TRY(hipMemcpyAsync(out, prof.p, sizeof(unsigned long long) * (size_t)n * kProfSlots,
@@ -752,7 +775,8 @@ clf_status fill_windows(clf_context* ctx, int32_t n, const uint8_t* const* event
     HIP_TRY(launch_zero_i32(b->dev.queue.as<int32_t>(), n + 1, st));
     SearchArgs a = b->dev.args(n);
     a.fill_only = mode;
-    HIP_TRY(launch_search(a, ctx->cfg.block_size, ctx->cfg.lds_hits, resolved_grid(ctx->cfg, ctx->n_cu, n), st));
+    const clf_launch_config lc = resolve_launch(ctx->cfg, b->total_hits, n);
+    HIP_TRY(launch_search(a, lc.block_size, lc.lds_hits, resolved_grid(lc, ctx->n_cu, n), st));
     std::vector<int4> cand((size_t)std::max<int64_t>(b->total_hits, 1));
     std::vector<EventResult> res(n);
     if (b->total_hits > 0)
@@ -915,8 +939,8 @@ clf_status clf_search_noalloc(clf_context* ctx, clf_mode mode, int32_t n,
         HIP_TRY(hipMemcpyAsync(r.desc.p, desc, sizeof(EventDesc) * (size_t)n, hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(r.order.p, order, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, st));
     }
-    clf_status s = enqueue_search(r, n, mode, ctx->cfg, ctx->n_cu, st, nullptr, nullptr, ctx->side,
-                                  ctx->fork, ctx->join);
+    clf_status s = enqueue_search(r, n, mode, resolve_launch(ctx->cfg, hits, n), ctx->n_cu, st, nullptr,
+                                  nullptr, ctx->side, ctx->fork, ctx->join);
     if (s != CLF_OK) return s;
     if (n == 0) {
         track_offset[0] = 0;

@@ -183,6 +183,14 @@ int context_device(const clf_context* ctx);
 // Persistent grid for n queued events under a launch configuration.
 int resolved_grid(const clf_launch_config& cfg, int n_cu, int n_items);
 
+// The launch a configuration gives a batch of n events with `hits` hits:
+// the configuration itself when block_size is set (lds_hits 0 meaning 256),
+// or, for block_size 0 (automatic), the tuned shape for the batch's mean
+// event size: 1024 threads with 512-hit slots above kAutoLargeEventHits
+// hits per event, else 512 threads with 256-hit slots.
+constexpr int64_t kAutoLargeEventHits = 4096;
+clf_launch_config resolve_launch(const clf_launch_config& cfg, int64_t hits, int32_t n);
+
 // Enqueues one search of n resident events on `stream`: queue reset, the
 // search kernel(s), track prefix and packing.  t0/t1 (nullable) are
 // recorded right around the search kernel(s).

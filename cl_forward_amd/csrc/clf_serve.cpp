@@ -512,7 +512,10 @@ clf_status launch_batch(clf_server* s, int k, bool device_idle, bool& launched) 
     HIP_TRY(hipMemcpyAsync(sl.dev.desc.p, d, sizeof(EventDesc) * (size_t)n, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(sl.dev.order.p, order, sizeof(int32_t) * (size_t)n,
                            hipMemcpyHostToDevice, st));
-    clf_status cs = enqueue_search(sl.dev, n, CLF_MODE_BATCHED, s->cfg, s->n_cu, st, nullptr, nullptr);
+    int64_t batch_hits = 0;
+    for (int i = 0; i < n; ++i) batch_hits += std::max(0, d[i].n_hits);
+    clf_status cs = enqueue_search(sl.dev, n, CLF_MODE_BATCHED, resolve_launch(s->cfg, batch_hits, n),
+                                   s->n_cu, st, nullptr, nullptr);
     if (cs != CLF_OK) return cs;
     HIP_TRY(hipMemcpyAsync(sl.h_results.p, sl.dev.results.p, sizeof(EventResult) * (size_t)n,
                            hipMemcpyDeviceToHost, st));
@@ -1404,10 +1407,14 @@ clf_status clf_server_create(clf_context* ctx, const clf_server_config* cfg, clf
     const int device = context_device(ctx);
     clf_server* s = new clf_server();
     s->ctx = ctx;
-    s->cfg = lc;
     s->n_cu = clf_context_num_cus(ctx);
     s->conf = c;
     s->resident = c.engine == CLF_SERVE_RESIDENT;
+    // the resident kernel's launch is fixed at creation: an automatic
+    // configuration takes the shape for events of a few thousand hits
+    // (512 x 256; 1024-thread workers served about 515k events/s against
+    // 705k); the batch engine resolves it per batch
+    s->cfg = s->resident ? resolve_launch(lc, 0, 0) : lc;
     s->device = device;
     int shift = 0;
     while ((1ll << shift) < c.max_requests) ++shift;

@@ -62,13 +62,20 @@ typedef struct clf_event_stats {
 } clf_event_stats;
 
 /* Launch parameters (no effect on results). */
+/* A context starts with block_size 0 and lds_hits 0: the launch shape is
+ * then chosen per batch from its mean event size (1024 threads and 512-hit
+ * slots above 4096 hits per event, else 512 threads and 256-hit slots; the
+ * resident serving kernel takes the latter).  Results never depend on the
+ * launch. */
 typedef struct clf_launch_config {
-    int32_t block_size; /* 64, 128, 256, 512 or 1024 threads per event workgroup */
+    int32_t block_size; /* 64, 128, 256, 512 or 1024 threads per event
+                           workgroup, or 0: chosen per batch               */
     int32_t grid_size;  /* persistent workgroups; 0 = CUs x occupancy        */
     int32_t grain;      /* consecutive queue items (events, in cost order)
                            a workgroup takes per dequeue, 1..64              */
     int32_t use_graph;  /* batched mode: capture and replay a hipGraph       */
-    int32_t lds_hits;   /* hits per LDS sensor slot, 256 or 512; a sensor
+    int32_t lds_hits;   /* hits per LDS sensor slot, 256 or 512, or 0: with
+                           the launch (256 for a set block_size); a sensor
                            with more hits is read from HBM                   */
     int32_t heavy_events; /* batched mode: the heavy_events costliest events
                            run in a second launch of heavy_block threads per
