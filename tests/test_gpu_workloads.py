@@ -7,6 +7,8 @@ byte for byte against the oracle through the C ABI.
   cost estimate plus a stride sample through two other launches;
 * stress1000 (Poisson(15) x Poisson(25), 5 % noise): the 16 costliest plus a
   stride sample through its tuned launch;
+* the automatic launch shape of a context that sets none, on small and on
+  large events;
 * the events whose 24-slot records overflow, worked out by hand, and the
   hand-worked candidate windows.
 """
@@ -121,6 +123,29 @@ def test_stress1000_costliest_and_sample(oracle_mod):
         ctx.set_config(**cfg)
         _say(f"stress1000: {len(blobs)} events (16 costliest + stride), tuned {ctx.config()}")
         _check(ctx, oracle_mod, blobs, "stress1000/tuned")
+
+
+def test_automatic_launch_shape(oracle_mod, batch1000):
+    """A context starts with block_size 0: the launch shape is chosen per
+    batch from its mean event size (above 4096 hits 1024 threads with
+    512-hit slots, else 512 with 256).  A batch of batch1000 events (mean
+    2.8k hits) and one of the 16 costliest stress1000 events (mean 12.4k)
+    both come out identical to the oracle, twice from their graphs; a set
+    block size with lds_hits 0 takes 256-hit slots."""
+    small = [batch1000[i] for i in costliest_and_stride(batch1000, n_top=2, n_stride=30)]
+    stress = generator.workload_blobs("stress1000")
+    big = [stress[i] for i in sorted(range(len(stress)), key=lambda i: -queue_cost(stress[i]))[:16]]
+    with Context(0) as ctx:
+        c = ctx.config()
+        assert c["block_size"] == 0 and c["lds_hits"] == 0, c
+        for name, blobs, large in (("small", small, False), ("large", big, True)):
+            mean = float(np.mean([struct.unpack_from("<ii", b, 0)[1] for b in blobs]))
+            assert (mean > 4096) == large, mean
+            _say(f"automatic launch: {len(blobs)} events of {mean:.0f} hits on average")
+            _check(ctx, oracle_mod, blobs, f"auto/{name}", reps=2)
+        ctx.set_config(block_size=256, lds_hits=0)
+        _say("block size 256, lds_hits 0")
+        _check(ctx, oracle_mod, small[:8], "block256/lds0")
 
 
 PLANES48 = [-300 + 22 * k for k in range(48)]
