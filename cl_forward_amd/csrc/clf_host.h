@@ -191,9 +191,11 @@ int resolved_grid(const clf_launch_config& cfg, int n_cu, int n_items);
 constexpr int64_t kAutoLargeEventHits = 4096;
 clf_launch_config resolve_launch(const clf_launch_config& cfg, int64_t hits, int32_t n);
 
-// Enqueues one search of n resident events on `stream`: queue reset, the
-// search kernel(s), track prefix and packing.  t0/t1 (nullable) are
-// recorded right around the search kernel(s).
+// Enqueues one search of n resident events on `stream`: the clear of the
+// queue counters and the output allocator, then the search kernel(s), which
+// pack each event's tracks (track_base[e], the `prefix` buffer, holds where
+// they start).  cfg must be resolved (resolve_launch).  t0/t1 (nullable)
+// are recorded right around the search kernel(s).
 // side/fork/join (nullable): the stream and events of the heavy-event
 // launch (cfg.heavy_events); without them everything runs in one launch.
 clf_status enqueue_search(const Resident& r, int32_t n, clf_mode mode,
