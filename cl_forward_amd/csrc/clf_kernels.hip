@@ -1189,12 +1189,18 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
     const SView& v4 = sc.v4;
 
     // windows of the next step (h2 windows of f-3, h1 windows of f-1),
-    // on the threads the forwarding leaves free (taken from the top)
+    // on the threads the forwarding leaves free (taken from the top); the
+    // h1 items start at a wave boundary, so that no wave runs both kinds
+    // (one that did would run the two chains one after the other)
     {
         const int g2 = f - 3, g1 = f - 1;
         const int nw2 = g2 >= 4 && f - 1 >= 4 ? n_w2_items(sh, g2) : 0;
         const int nw1 = n_w1_items(sh, g1);
-        for (int i = B - 1 - tid; i < nw2 + nw1; i += B) window_item<L>(sh, ev, i, g2, nw2, g1);
+        const int w2_slots = (nw2 + kWave - 1) / kWave * kWave;
+        for (int j = B - 1 - tid; j < w2_slots + nw1; j += B) {
+            if (j < nw2) window_item<L>(sh, ev, j, g2, nw2, g1);
+            else if (j >= w2_slots) window_item<L>(sh, ev, nw2 + (j - w2_slots), g2, nw2, g1);
+        }
     }
 
     // (c) forwarding of the ttf entries of step f+1 (Kernel.cl:210-367)
@@ -1575,7 +1581,10 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
         const int nw2 = n_w2_items(sh, S - 3), nw1 = n_w1_items(sh, S - 1);
         bool lds = true;
         for (int s2 = S - 1; s2 >= S - 5 && s2 >= 0; --s2) lds = lds && uni(sh.s_num[s2]) <= C;
-        for (int i = tid; i < nw2 + nw1; i += B) {
+        const int w2_slots = (nw2 + kWave - 1) / kWave * kWave;  // h1 items from a wave boundary
+        for (int j = tid; j < w2_slots + nw1; j += B) {
+            if (j >= nw2 && j < w2_slots) continue;
+            const int i = j < nw2 ? j : nw2 + (j - w2_slots);
             if (lds) window_item<true>(sh, ev, i, S - 3, nw2, S - 1);
             else window_item<false>(sh, ev, i, S - 3, nw2, S - 1);
         }
