@@ -124,6 +124,8 @@ SearchArgs Resident::args(int32_t n) const {
     a.ttf_a = ttf_a.as<TtfEntry>();
     a.ttf_b = ttf_b.as<TtfEntry>();
     a.weak = weak.as<uint4>();
+    a.slot_records = 0;
+    a.slot_base = 0;
     a.results = results.as<EventResult>();
     // prefix[e]: first packed record of event e; prefix[n]: the allocator,
     // which ends as the number of packed records
@@ -161,6 +163,10 @@ clf_launch_config resolve_launch(const clf_launch_config& cfg, int64_t hits, int
     return r;
 }
 
+int64_t slot_records(const Resident& r, int64_t slots) {
+    return r.max_slice > 0 && slots * r.max_slice <= r.work_records ? r.max_slice : 0;
+}
+
 clf_status enqueue_search(const Resident& r, int32_t n, clf_mode mode,
                           const clf_launch_config& cfg, int n_cu, hipStream_t st, hipEvent_t t0,
                           hipEvent_t t1, hipStream_t side, hipEvent_t fork, hipEvent_t join) {
@@ -178,6 +184,9 @@ clf_status enqueue_search(const Resident& r, int32_t n, clf_mode mode,
         // the main launch; joined before packing
         HIP_TRY(hipEventRecord(fork, st));
         HIP_TRY(hipStreamWaitEvent(side, fork, 0));
+        // work slots: the heavy launch's first, then the main launch's
+        const int grid = heavy < n ? resolved_grid(cfg, n_cu, n - heavy) : 0;
+        a.slot_records = slot_records(r, (int64_t)heavy + grid);
         SearchArgs h = a;
         h.first_item = 0;
         h.n_items = heavy;
@@ -187,16 +196,20 @@ clf_status enqueue_search(const Resident& r, int32_t n, clf_mode mode,
         if (heavy < n) {
             a.first_item = heavy;
             a.n_items = n - heavy;
-            HIP_TRY(launch_search(a, cfg.block_size, cfg.lds_hits,
-                                  resolved_grid(cfg, n_cu, n - heavy), st));
+            a.slot_base = heavy;
+            HIP_TRY(launch_search(a, cfg.block_size, cfg.lds_hits, grid, st));
         }
         HIP_TRY(hipStreamWaitEvent(st, join, 0));
     } else if (mode == CLF_MODE_BATCHED) {
-        HIP_TRY(launch_search(a, cfg.block_size, cfg.lds_hits, resolved_grid(cfg, n_cu, n), st));
+        const int grid = resolved_grid(cfg, n_cu, n);
+        a.slot_records = slot_records(r, grid);
+        HIP_TRY(launch_search(a, cfg.block_size, cfg.lds_hits, grid, st));
     } else {
-        // plain: event i alone, in its own launch of one workgroup
+        // plain: event i alone, in its own launch of one workgroup; the
+        // launches run one after another on the stream and share one slot
         a.order = nullptr;
         a.n_items = 1;
+        a.slot_records = slot_records(r, 1);
         for (int i = 0; i < n; ++i) {
             a.first_item = i;
             a.queue_head = r.queue.as<int32_t>() + 1 + i;
@@ -539,7 +552,7 @@ clf_status batch_create(clf_context* ctx, int32_t n, const uint8_t* const* event
     b->h_desc.resize(n);
     std::vector<double> cost(n, 0.0);
     size_t off = 0;
-    int64_t hits = 0, cap = 0;
+    int64_t hits = 0, cap = 0, max_slice = 0;
     for (int i = 0; i < n; ++i) {
         int32_t S = 0, N = 0;
         EventDesc& d = b->h_desc[i];
@@ -552,6 +565,7 @@ clf_status batch_create(clf_context* ctx, int32_t n, const uint8_t* const* event
             off += align_up(8 + 12 * (size_t)S + 16 * (size_t)N, 256);
             hits += N;
             cap += track_slice(N, S);
+            max_slice = std::max(max_slice, track_slice(N, S));
             cost[i] = event_cost(events[i], S, N);
         } else {
             d.blob_off = 0;
@@ -575,6 +589,8 @@ clf_status batch_create(clf_context* ctx, int32_t n, const uint8_t* const* event
     };
     chk(b->h_blobs.ensure(off));
     chk(b->dev.ensure(n, off, hits, cap));
+    b->dev.work_records = cap;
+    b->dev.max_slice = max_slice;
     chk(b->h_results.ensure(sizeof(EventResult) * (size_t)n));
     chk(b->h_prefix.ensure(sizeof(int64_t) * ((size_t)n + 1)));
     if (e != hipSuccess) {
@@ -709,10 +725,13 @@ clf_status clf_batch_profile(clf_context* ctx, clf_batch* b, clf_mode mode, uint
     const clf_launch_config lc = resolve_launch(ctx->cfg, b->total_hits, n);
     HIP_TRY(launch_zero_i32(b->dev.queue.as<int32_t>(), n + 1, st, a.track_alloc));
     if (mode == CLF_MODE_BATCHED) {
-        HIP_TRY(launch_search(a, lc.block_size, lc.lds_hits, resolved_grid(lc, ctx->n_cu, n), st));
+        const int grid = resolved_grid(lc, ctx->n_cu, n);
+        a.slot_records = slot_records(b->dev, grid);
+        HIP_TRY(launch_search(a, lc.block_size, lc.lds_hits, grid, st));
     } else {
         a.order = nullptr;
         a.n_items = 1;
+        a.slot_records = slot_records(b->dev, 1);
         for (int i = 0; i < n; ++i) {
             a.first_item = i;
             a.queue_head = b->dev.queue.as<int32_t>() + 1 + i;
@@ -874,7 +893,7 @@ clf_status clf_search_noalloc(clf_context* ctx, clf_mode mode, int32_t n,
     int32_t* order = ctx->na_order.data();
     double* cost = ctx->na_cost.data();
     size_t off = 0;
-    int64_t hits = 0, cap = 0;
+    int64_t hits = 0, cap = 0, max_slice = 0;
     for (int i = 0; i < n; ++i) {
         int32_t S = 0, N = 0;
         EventDesc& d = desc[i];
@@ -888,6 +907,7 @@ clf_status clf_search_noalloc(clf_context* ctx, clf_mode mode, int32_t n,
             off += align_up(event_bytes(S, N), 256);
             hits += N;
             cap += track_slice(N, S);
+            max_slice = std::max(max_slice, track_slice(N, S));
             cost[i] = event_cost(events[i], S, N);
         } else {
             d.blob_off = 0;
@@ -928,6 +948,8 @@ clf_status clf_search_noalloc(clf_context* ctx, clf_mode mode, int32_t n,
     const size_t offs[] = {o_blobs, o_desc,  o_order, o_queue,   o_cand,   o_used,  o_tracks,
                            o_ttf_a, o_ttf_b, o_weak,  o_results, o_prefix, o_packed};
     for (int k = 0; k < 13; ++k) bufs[k]->p = base + offs[k];
+    r.work_records = cap;
+    r.max_slice = max_slice;
     hipStream_t st = ctx->stream;
     for (int i = 0; i < n; ++i) {
         if (desc[i].n_hits < 0) continue;

@@ -65,10 +65,17 @@ struct SearchArgs {
                            // check; otherwise W1Rec[N] then W2Rec[N] of
                            // sensors too large for an LDS slot
     uint8_t* used;         // per hit (events above the LDS bitmask)
-    clf_track* tracks;     // per event, track_off records (the output)
-    TtfEntry* ttf_a;       // per event, track_off entries
+    // Work slices: the tracks being built, the two ttf lists and the weak
+    // tracklets' hits, each a slice of records at one offset.  With
+    // slot_records > 0 a workgroup's slice is slot slot_base + blockIdx.x
+    // (slot_records records each), reused by every event the workgroup
+    // takes; with 0 each event has its own at its descriptor's track_off.
+    clf_track* tracks;
+    TtfEntry* ttf_a;
     TtfEntry* ttf_b;
-    uint4* weak;           // per event, track_off entries: tracklet hits
+    uint4* weak;
+    int64_t slot_records;
+    int32_t slot_base;
     EventResult* results;  // per event
     // Packed output (nullptr: none, the caller copies from `tracks`): at the
     // end of event e its workgroup reserves n_tracks records at

@@ -170,6 +170,9 @@ double event_cost(const uint8_t* b, int32_t S, int32_t N);
 struct Resident {
     DevBuf blobs, desc, order, queue, cand, used, tracks, ttf_a, ttf_b, weak, results, prefix,
         packed;
+    // the batch these buffers hold: the records of its per-event work
+    // slices (the capacity `ensure` was given) and its largest slice
+    int64_t work_records = 0, max_slice = 0;
     // n events, blob_bytes of packed blobs, hits in total, track capacity
     hipError_t ensure(int32_t n, size_t blob_bytes, int64_t hits, int64_t cap);
     void release();
@@ -196,6 +199,13 @@ clf_launch_config resolve_launch(const clf_launch_config& cfg, int64_t hits, int
 // pack each event's tracks (track_base[e], the `prefix` buffer, holds where
 // they start).  cfg must be resolved (resolve_launch).  t0/t1 (nullable)
 // are recorded right around the search kernel(s).
+// Records per work slot when `slots` workgroups each take one slot of the
+// batch's largest slice and the slots fit where its per-event slices lie
+// (every workgroup then reuses its slot, event after event, so the slices'
+// lines stay in L2 between events); 0 when they do not fit (one slice per
+// event).
+int64_t slot_records(const Resident& r, int64_t slots);
+
 // side/fork/join (nullable): the stream and events of the heavy-event
 // launch (cfg.heavy_events); without them everything runs in one launch.
 clf_status enqueue_search(const Resident& r, int32_t n, clf_mode mode,

@@ -1406,6 +1406,13 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
 // ---------------------------------------------------------------------------
 // One event
 // ---------------------------------------------------------------------------
+// First record of the work slices of the event the workgroup runs: its
+// workgroup's slot, reused event after event, or its own slice.
+__device__ __forceinline__ int64_t work_offset(const SearchArgs& a, int64_t event_off) {
+    return a.slot_records > 0 ? (int64_t)(a.slot_base + (int)blockIdx.x) * a.slot_records
+                              : uni(event_off);
+}
+
 template <int B, int C>
 __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
     constexpr int W = Shared<B, C>::W;
@@ -1416,7 +1423,7 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
     EventDesc d;
     d.blob_off = uni(dv.blob_off);
     d.hit_off = uni(dv.hit_off);
-    d.track_off = uni(dv.track_off);
+    d.track_off = work_offset(a, dv.track_off);
     d.n_hits = uni(dv.n_hits);
     d.n_sensors = uni(dv.n_sensors);
     EventResult* res = a.results + e;
@@ -1706,7 +1713,7 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
         __syncthreads();  // also orders the weak filter's track stores before the reads
         const int64_t base = uni(sh.pack_base);
         const int nt = uni(sh.item);
-        const int64_t toff = uni(a.ev[e].track_off);
+        const int64_t toff = work_offset(a, a.ev[e].track_off);
         // consecutive lanes write consecutive words; the zero padding is
         // made here (track_word)
         const uint32_t* src = reinterpret_cast<const uint32_t*>(a.tracks + toff);

@@ -481,7 +481,7 @@ clf_status launch_batch(clf_server* s, int k, bool device_idle, bool& launched) 
     Run runs[4];
     int n_runs = 0;
     size_t pos = 0, prev_end = (size_t)-1;
-    int64_t hit_off = 0, track_off = 0;
+    int64_t hit_off = 0, track_off = 0, max_slice = 0;
     std::vector<double> cost(n);
     for (int i = 0; i < n; ++i) {
         const Request& r = s->reqs[sl.reqs[i]];
@@ -499,8 +499,11 @@ clf_status launch_batch(clf_server* s, int k, bool device_idle, bool& launched) 
         prev_end = r.in_off + r.span;
         hit_off += r.N;
         track_off += r.cap;
+        max_slice = std::max(max_slice, r.cap);
         cost[i] = r.cost;
     }
+    sl.dev.work_records = s->slot_cap;  // what the slot's buffers hold
+    sl.dev.max_slice = max_slice;
     std::iota(order, order + n, 0);
     std::stable_sort(order, order + n, [&](int x, int y) { return cost[x] > cost[y]; });
 
@@ -685,6 +688,8 @@ ServeArgs serve_args(clf_server* s) {
     b.ttf_a = s->wg_ttf_a.as<TtfEntry>();
     b.ttf_b = s->wg_ttf_b.as<TtfEntry>();
     b.weak = s->wg_weak.as<uint4>();
+    b.slot_records = 0;  // each worker's slices: its descriptor's track_off
+    b.slot_base = 0;
     b.results = s->wg_results.as<EventResult>();
     b.packed = nullptr;  // serve_one copies the tracks to the request itself
     b.track_base = nullptr;

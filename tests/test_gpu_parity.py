@@ -145,6 +145,31 @@ def test_heavy_lane_identical(oracle_mod, config_blobs, heavy, graph):
         assert again == [oracle_mod.search_event(x)[0] for x in config_blobs]
 
 
+@pytest.mark.parametrize("grid", [2, 8])
+def test_work_slots_and_event_slices(oracle_mod, grid):
+    """Work slices per workgroup slot (reused event after event) when the
+    slots fit in the batch's per-event slices, else one slice per event:
+    one stress event beside seven small ones gives both (2 slots of its
+    slice fit, 8 do not), each run three times over the same buffers."""
+    from cl_forward_amd.search import Context, track_capacity
+
+    blobs = [generator.stress_event(3).pack()]
+    blobs += [generator.generate_event(500 + i, n_tracks=4, noise_fraction=0.0).pack() for i in range(7)]
+    slices = [(track_capacity(b) + 3) // 4 * 4 for b in blobs]
+    fits = grid * max(slices) <= sum(slices)
+    assert fits == (grid == 2), (grid, slices)
+    _say(f"search: grid {grid}, work slots {'fit' if fits else 'do not fit'}")
+    want = [oracle_mod.search_event(x)[0] for x in blobs]
+    with Context(0) as ctx:
+        ctx.set_config(block_size=512, lds_hits=256, grid_size=grid)
+        _check_search(ctx, oracle_mod, blobs, f"grid{grid}")
+        with ctx.batch(blobs) as b:
+            for rep in range(3):
+                b.run(MODE_BATCHED)
+                outs, _ = b.results()
+                assert outs == want, f"run {rep}"
+
+
 def test_determinism_and_graph_replays(gpu_ctx, plain51_blobs):
     _say("search: repeated graph replays give identical bytes")
     with gpu_ctx.batch(plain51_blobs) as b:
