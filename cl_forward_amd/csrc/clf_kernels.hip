@@ -117,6 +117,15 @@ __device__ __forceinline__ int kth_set_bit(uint64_t m, int k) {
 // Values every lane of the wave holds alike (LDS words read at a uniform
 // address, block totals), moved to scalar registers so that they take no
 // vector registers.  Only for values that are the same in all active lanes.
+// A workgroup barrier whose fences cover LDS only: it waits for the
+// wave's LDS accesses (lgkmcnt) but leaves its global stores in flight,
+// which a full barrier would wait for.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
 __device__ __forceinline__ float uni(float x) {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
@@ -180,6 +189,12 @@ struct StageThreads {
     static constexpr int count = B - first;
     static constexpr bool during_gather = first > 0;
 };
+// Configurations whose sensor f-6 is loaded into registers at the top of
+// step f and stored into the ring during the step (in the forwarding's last
+// tile, or during the gather when there is no forwarding).
+template <int B, int C>
+constexpr bool kStageInForward = StageThreads<B, C>::during_gather && B <= 512;
+
 template <int B, int C>
 struct Ring {
     static constexpr int slots = StageThreads<B, C>::during_gather ? kStepSensors + 1 : kStepSensors;
@@ -220,6 +235,9 @@ struct SharedCore {
     // gather state of the current sensor step (Kernel.cl:720-721,753-754)
     uint32_t g_prev, g_shift, g_counter;
     int32_t g_nv, g_more;
+    // set when this step's forwarding marks a hit of the gather sensor f
+    // (the first batch, gathered during the forwarding, is then redone)
+    int32_t g_redo;
     int32_t n_listed;  // entries of the list the next step forwards
     int64_t pack_base;  // first packed record of the current event
     // the current batch of gathered h0 (indices relative to sensor f)
@@ -653,9 +671,16 @@ struct UsedFlags {
 // Visits (relative to sensor f) are appended to v_h0 in lane order.
 // ---------------------------------------------------------------------------
 template <int B, int C>
-__device__ int gather_batch(Shared<B, C>& sh, const UsedFlags& used, int start0, uint32_t hn0) {
+__device__ int gather_batch(Shared<B, C>& sh, const UsedFlags& used, int start0, uint32_t hn0,
+                            bool fresh) {
     const int lane = lane_id();
-    uint32_t prev = sh.g_prev, shift = sh.g_shift, counter = sh.g_counter;
+    // fresh: the step's first batch, from the start of the sensor
+    uint32_t prev = 0u, shift = CLF_GATHER_CHUNK, counter = 0u;
+    if (!fresh) {
+        prev = sh.g_prev;
+        shift = sh.g_shift;
+        counter = sh.g_counter;
+    }
     int nv = 0;
     while (prev < hn0 && nv + kWave <= kMaxVisits) {
         const uint32_t e = prev + (uint32_t)lane;
@@ -1187,6 +1212,24 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
     sc.z4 = sc.fast4 && sc.v4.zmin == sc.v4.zmax && sc.v4.zmin != 0.f;
     sc.seeds = f - 2 >= 4;
     const SView& v4 = sc.v4;
+    // the seeding's sensors and window tables
+    auto seed_views = [&]() {
+        sc.v0 = sensor_view<L>(sh, ev, f);
+        sc.v2 = sensor_view<L>(sh, ev, f - 2);
+        sc.w1 = w1_of<L>(sh, ev, f);
+        sc.w2 = w2_of<L>(sh, ev, f - 2);
+        const bool marks = f - 2 >= 0 && (L || sh.s_num[f - 2] <= C);
+        sc.w2_marks = marks ? sh.w2_empty[(f - 2) & 1] : nullptr;
+        sc.w2_n_empty = &sh.w2_n_empty[(f - 2) & 1];
+    };
+    // When the forwarding fits on waves 1.. (one tile, entry i on thread
+    // i + 64), wave 0 gathers and sets up the step's first seeding batch
+    // while the other waves forward.  The gather reads the used flags of
+    // sensor f; the forwarding marks a hit there only when a tracklet that
+    // skipped one or three sensors becomes a track, and then sets g_redo
+    // and wave 0 gathers again after the forwarding.  Nothing else the
+    // gather or the setup reads is written during the forwarding.
+    const bool early = B > kWave && n_cur <= B - kWave;
 
     // windows of the next step (h2 windows of f-3, h1 windows of f-1),
     // on the threads the forwarding leaves free (taken from the top); the
@@ -1201,6 +1244,12 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
             if (j < nw2) window_item<L>(sh, ev, j, g2, nw2, g1);
             else if (j >= w2_slots) window_item<L>(sh, ev, nw2 + (j - w2_slots), g2, nw2, g1);
         }
+    }
+
+    if (early && wave == 0) {
+        seed_views();
+        const int nv0 = gather_batch<B, C>(sh, used, sc.v0.start, (uint32_t)sc.v0.n, true);
+        if (nv0 > 0) seed_prepare<B, C>(sh, sc, nv0);
     }
 
     // (c) forwarding of the ttf entries of step f+1 (Kernel.cl:210-367)
@@ -1288,10 +1337,19 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
                 used.set((int)t.t0);
                 used.set((int)t.t1);
                 used.set((int)t.t2);
+                // sensor f's range read here, not held across the loop
+                const uint32_t fs = (uint32_t)sh.s_start[f], fn = (uint32_t)sh.s_num[f];
+                if (t.t0 - fs < fn || t.t1 - fs < fn || t.t2 - fs < fn) sh.g_redo = 1;
             }
         }
         int pos[3], tot[3];
         block_scan3<B, C>(sh, f_new, f_req, f_weak, pos, tot, st.sel);
+        // the staged sensor f-6 goes to its ring slot (that of f+1, which
+        // no part of this step reads) before the last tile's global stores:
+        // the vector-memory counter retires loads and stores in issue
+        // order, so the wait for the staged loads at the store would
+        // otherwise wait for those stores too
+        if (kStageInForward<B, C> && base + B >= n_cur) stage_store<B, C>(sh, staged);
         if (active) {
             if (found && !f_req) {
                 // a full record found a 25th hit: the record stays as stored
@@ -1339,18 +1397,7 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
         n_next += tot[1];
         n_weak += tot[2];
     }
-    if (tid == 0) {
-        sh.g_prev = 0;
-        sh.g_shift = CLF_GATHER_CHUNK;
-        sh.g_counter = 0;
-    }
-    sc.v0 = sensor_view<L>(sh, ev, f);
-    sc.v2 = sensor_view<L>(sh, ev, f - 2);
-    sc.w1 = w1_of<L>(sh, ev, f);
-    sc.w2 = w2_of<L>(sh, ev, f - 2);
-    const bool marks = f - 2 >= 0 && (L || sh.s_num[f - 2] <= C);
-    sc.w2_marks = marks ? sh.w2_empty[(f - 2) & 1] : nullptr;
-    sc.w2_n_empty = &sh.w2_n_empty[(f - 2) & 1];
+    seed_views();
     // No barrier here: every used flag of the forwarding was set before the
     // last block scan's barrier, and what the forwarding writes after it
     // (tracks, ttf entries of the next list, weak) is not read by the gather
@@ -1369,17 +1416,29 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
     bool first_batch = true;
     for (;;) {
         if (wave == 0) {
-            const int nv0 = gather_batch<B, C>(sh, used, start0, hn0);
-            prof_mark(sh, pc, PROF_GATHER);
-            if (nv0 > 0) seed_prepare<B, C>(sh, sc, nv0);
+            // the first batch stands if it was gathered during the
+            // forwarding and no flag of sensor f changed meanwhile (g_redo
+            // is set before the forwarding scan's barrier)
+            if (!(first_batch && early && uni(sh.g_redo) == 0)) {
+                const int nv0 = gather_batch<B, C>(sh, used, start0, hn0, first_batch);
+                prof_mark(sh, pc, PROF_GATHER);
+                if (nv0 > 0) seed_prepare<B, C>(sh, sc, nv0);
+            }
         } else if (StageThreads<B, C>::during_gather && first_batch) {
-            // sensor f-6, for the next step: from registers loaded a step
-            // ahead, or (1024 threads) straight from HBM
-            if (B <= 512) stage_store<B, C>(sh, staged);
-            else stage_sensor<B, C, StageThreads<B, C>::first>(sh, ev, f - 6);
+            // sensor f-6, for the next step: from registers loaded at the
+            // top of the step (stored in the forwarding when it has entries),
+            // or (1024 threads) straight from HBM
+            if (!kStageInForward<B, C>) stage_sensor<B, C, StageThreads<B, C>::first>(sh, ev, f - 6);
+            else if (n_cur == 0) stage_store<B, C>(sh, staged);
         }
         first_batch = false;
-        __syncthreads();
+        // what crosses this barrier is in LDS (the batch's visit tables, the
+        // staged ring slot, g_nv and g_more); the forwarding's global stores
+        // (tracks, weak and ttf entries beyond the lists' LDS parts) are read
+        // or rewritten only after the pair fits' full barrier or the next
+        // step's, which wait for them, and the used flags in HBM were set
+        // before the forwarding scan's barrier
+        lds_barrier();
         prof_mark(sh, pc, PROF_SEED_SETUP);
         const int nv = uni(sh.g_nv);
         const bool more = uni(sh.g_more) != 0;
@@ -1616,6 +1675,7 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
             else stage_sensor<B, C>(sh, ev, f - 5);
         }
         clear_empty_marks<B, C>(sh, (f - 3) & 1);  // filled by this step's windows of f-3
+        if (tid == 0) sh.g_redo = 0;
         prof_mark(sh, pc, PROF_STAGE);
         __syncthreads();
         n_cur = uni(sh.n_listed);
