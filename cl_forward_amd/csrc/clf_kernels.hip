@@ -1331,17 +1331,6 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
         // used flags before the scan's barrier: the gather and the seeding
         // of this step read them after it (a tracklet that skipped sensors
         // marks hits of sensors f and f-2 when it becomes a track)
-        // up to 512 threads, the found hit's coordinates for the new ttf
-        // entry are read before the scan's barrier, so that their latency
-        // passes in its wait (1024-thread workgroups have no registers to
-        // hold them across it)
-        constexpr bool kEarlyHit = B <= 512;
-        float bx1 = 0.f, by1 = 0.f, bz1 = 0.f;
-        if (kEarlyHit && found) {
-            bx1 = v4.x[bk];
-            by1 = v4.y[bk];
-            bz1 = v4.z[bk];
-        }
         if (found) {
             used.set(v4.start + bk);
             if (f_new) {
@@ -1390,9 +1379,9 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
                 n.x0 = t.x1;
                 n.y0 = t.y1;
                 n.z0 = t.z1;
-                n.x1 = kEarlyHit ? bx1 : v4.x[bk];
-                n.y1 = kEarlyHit ? by1 : v4.y[bk];
-                n.z1 = kEarlyHit ? bz1 : v4.z[bk];
+                n.x1 = v4.x[bk];
+                n.y1 = v4.y[bk];
+                n.z1 = v4.z[bk];
                 n.pad = 0;
                 ttf_put(sh, nxt, n_next + pos[1], n);
             } else if (f_req) {
