@@ -1088,11 +1088,13 @@ struct StagedSensor {
     float x_first, x_last;
     int s;  // the sensor, or -1 (nothing staged: absent or read from HBM)
 };
+// first: the first staging thread (hit r * kThreads + t on thread first + t)
 template <int B, int C>
 __device__ __forceinline__ void stage_load(const Shared<B, C>& sh, const EventView& ev, int s,
-                                           StagedSensor<B, C>& g) {
+                                           StagedSensor<B, C>& g,
+                                           int first = StageThreads<B, C>::first) {
     g.s = -1;
-    const int t = (int)threadIdx.x - StageThreads<B, C>::first;
+    const int t = (int)threadIdx.x - first;
     if (s < 0 || t < 0) return;
     const int n = uni(sh.s_num[s]);
     if (n > C) return;
@@ -1112,10 +1114,11 @@ __device__ __forceinline__ void stage_load(const Shared<B, C>& sh, const EventVi
     g.x_last = n > 1 ? ev.X[b + n - 1] : 0.f;
 }
 template <int B, int C>
-__device__ __forceinline__ void stage_store(Shared<B, C>& sh, const StagedSensor<B, C>& g) {
+__device__ __forceinline__ void stage_store(Shared<B, C>& sh, const StagedSensor<B, C>& g,
+                                            int first = StageThreads<B, C>::first) {
     const int s = g.s;
     if (s < 0) return;  // also every thread outside the staging threads
-    const int t = (int)threadIdx.x - StageThreads<B, C>::first;
+    const int t = (int)threadIdx.x - first;
     const int n = uni(sh.s_num[s]);
     const int k = s % Ring<B, C>::slots;
     const bool sorted = uni((int)sh.s_sorted[s]) != 0;
@@ -1191,7 +1194,8 @@ template <bool L, int B, int C, bool EarlySeed>
 __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
                                          const UsedFlags& used, clf_track* tracks, uint4* weak,
                                          bool degenerate, int f, StepState& st,
-                                         const StagedSensor<B, C>& staged, PhaseClock& pc) {
+                                         const StagedSensor<B, C>& staged, int stage_first,
+                                         PhaseClock& pc) {
     const int tid = threadIdx.x, wave = wave_id();
     int& n_cur = st.n_cur;
     int& n_tracks = st.n_tracks;
@@ -1251,6 +1255,9 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
         const int nv0 = gather_batch<B, C>(sh, used, sc.v0.start, (uint32_t)sc.v0.n, true);
         if (nv0 > 0) seed_prepare<B, C>(sh, sc, nv0);
     }
+    // stage_first > 0: the staged sensor goes to its slot now, from waves
+    // that have no forwarding entries or window items in this step
+    if (kStageInForward<B, C> && stage_first > 0) stage_store<B, C>(sh, staged, stage_first);
 
     // (c) forwarding of the ttf entries of step f+1 (Kernel.cl:210-367)
     int n_next = 0;
@@ -1349,7 +1356,7 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
         // the vector-memory counter retires loads and stores in issue
         // order, so the wait for the staged loads at the store would
         // otherwise wait for those stores too
-        if (kStageInForward<B, C> && base + B >= n_cur) stage_store<B, C>(sh, staged);
+        if (kStageInForward<B, C> && stage_first == 0 && base + B >= n_cur) stage_store<B, C>(sh, staged);
         if (active) {
             if (found && !f_req) {
                 // a full record found a 25th hit: the record stays as stored
@@ -1429,7 +1436,7 @@ __device__ __forceinline__ void run_step(Shared<B, C>& sh, const EventView& ev,
             // top of the step (stored in the forwarding when it has entries),
             // or (1024 threads) straight from HBM
             if (!kStageInForward<B, C>) stage_sensor<B, C, StageThreads<B, C>::first>(sh, ev, f - 6);
-            else if (n_cur == 0) stage_store<B, C>(sh, staged);
+            else if (n_cur == 0 && stage_first == 0) stage_store<B, C>(sh, staged);
         }
         first_batch = false;
         // what crosses this barrier is in LDS (the batch's visit tables, the
@@ -1473,12 +1480,13 @@ __device__ __forceinline__ int64_t work_offset(const SearchArgs& a, int64_t even
 }
 
 // EarlySeed: wave 0 gathers a step's first seeding batch during the
-// forwarding (the batch kernel).  The serving kernel does not: it holds
-// its worker state across the search, and with the early gather its
-// spill space grew from 144 to 672 bytes per lane and each search took
-// 422 us instead of 329 us on the serve256 events.
+// forwarding, and idle waves store the staged sensor before it (the batch
+// kernel).  The serving kernel does neither: it holds its worker state
+// across the search, and with the early gather its spill space grew from
+// 144 to 672 bytes per lane and each search took 422 us instead of 329 us
+// on the serve256 events (the staging by idle waves, 192 bytes and 334 us).
 template <int B, int C, bool EarlySeed>
-__device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
+__device__ __forceinline__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
     constexpr int W = Shared<B, C>::W;
     const int tid = threadIdx.x, lane = lane_id(), wave = wave_id();
     // the descriptor, in scalar registers: the sensor loop below then runs
@@ -1685,9 +1693,27 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
         __syncthreads();
         n_cur = uni(sh.n_listed);
         // sensor f-6 for the next step, stored at the top of the next step
-        // into the slot of sensor f, or during this step's gather into that
-        // of f+1 (seven slots)
-        if (kAhead) stage_load<B, C>(sh, ev, f - 6, staged);
+        // into the slot of sensor f, or during this step into that of f+1
+        // (seven slots): by waves that neither forward nor compute windows
+        // in this step, before the forwarding, when enough of them are
+        // free (one hit each); else by waves 1.. in the forwarding's last
+        // tile
+        int stage_first = 0;  // > 0: the staging threads start there and store before the forwarding
+        if (kStageInForward<B, C>) {
+            if (EarlySeed && f - 6 >= 0 && n_cur <= B - kWave) {
+                const int g2 = f - 3, g1 = f - 1;
+                const int nw2 = g2 >= 4 && g1 >= 4 ? n_w2_items(sh, g2) : 0;
+                const int items = (nw2 + kWave - 1) / kWave * kWave + n_w1_items(sh, g1);
+                const int lo = kWave + (n_cur + kWave - 1) / kWave * kWave;
+                const int hi = B - (items + kWave - 1) / kWave * kWave;
+                // at least one thread: thread `first` writes the slot's
+                // bucket scales (and, for an empty sensor, its starts)
+                if (hi - lo >= max(uni(sh.s_num[f - 6]), 1)) stage_first = lo;
+            }
+            stage_load<B, C>(sh, ev, f - 6, staged, stage_first > 0 ? stage_first : StageThreads<B, C>::first);
+        } else if (kAhead) {
+            stage_load<B, C>(sh, ev, f - 6, staged);
+        }
         prof_mark(sh, pc, PROF_STAGE_WAIT);
         prof_add(sh, pc, CNT_FWD_ENTRIES, (unsigned long long)n_cur);
         prof_add(sh, pc, CNT_FWD_TILES, (unsigned long long)((n_cur + B - 1) / B));
@@ -1698,9 +1724,9 @@ __device__ void process_event(const SearchArgs& a, int e, Shared<B, C>& sh) {
 #pragma unroll
         for (int d = 0; d < kStepSensors; ++d) lds_step &= uni(sh.s_num[max(f - d, 0)]) <= C;
         if (lds_step)
-            run_step<true, B, C, EarlySeed>(sh, ev, used, tracks, weak, degenerate, f, st, staged, pc);
+            run_step<true, B, C, EarlySeed>(sh, ev, used, tracks, weak, degenerate, f, st, staged, stage_first, pc);
         else
-            run_step<false, B, C, EarlySeed>(sh, ev, used, tracks, weak, degenerate, f, st, staged, pc);
+            run_step<false, B, C, EarlySeed>(sh, ev, used, tracks, weak, degenerate, f, st, staged, stage_first, pc);
         n_cur = st.n_cur;
         n_tracks = st.n_tracks;
         n_tracklets = st.n_tracklets;
