@@ -43,6 +43,8 @@
 #include <thread>
 #include <vector>
 
+#include <immintrin.h>
+
 #include <hsa/hsa.h>
 #include <hsa/hsa_ext_amd.h>
 
@@ -1182,11 +1184,22 @@ clf_status create_batches(clf_server* s) {
 
 // Writes the words of an event blob that the search reads, packed
 // (SearchedWords), to dst: what the resident engine stages and uploads.
+// 16-byte words to a 16-byte aligned staging destination with
+// non-temporal stores: the copy engine reads the staging arena soon after,
+// and lines left dirty in the host's caches would have to be snooped
+void stream_copy16(uint8_t* dst, const uint8_t* src, size_t n16) {
+    __m128i* d = reinterpret_cast<__m128i*>(dst);
+    const __m128i* q = reinterpret_cast<const __m128i*>(src);
+    for (size_t i = 0; i < n16; ++i) _mm_stream_si128(d + i, _mm_loadu_si128(q + i));
+}
+// The words the search reads (header and sensor table, then the
+// coordinates from the word holding X[0], then the last bytes), packed; the
+// fence orders the non-temporal stores before the request's publication.
 void stage_searched(uint8_t* dst, const uint8_t* blob, const SearchedWords& w) {
-    std::memcpy(dst, blob, 16 * (size_t)std::min(w.h16, w.m16));
-    if (w.m16 > w.h16)
-        std::memcpy(dst + 16 * w.h16, blob + 16 * w.x16, 16 * (size_t)(w.m16 - w.h16));
+    stream_copy16(dst, blob, (size_t)std::min(w.h16, w.m16));
+    if (w.m16 > w.h16) stream_copy16(dst + 16 * w.h16, blob + 16 * w.x16, (size_t)(w.m16 - w.h16));
     std::memcpy(dst + 16 * w.m16, blob + 16 * w.n16, (size_t)(w.bytes - 16 * w.n16));
+    _mm_sfence();
 }
 
 // Submits events[0..n) in order and stops at the first one that is
