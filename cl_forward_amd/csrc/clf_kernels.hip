@@ -1090,15 +1090,11 @@ struct StagedSensor {
 };
 // first: the first staging thread (hit r * kThreads + t on thread first + t)
 template <int B, int C>
-__device__ __forceinline__ void stage_load(const Shared<B, C>& sh, const EventView& ev, int s,
-                                           StagedSensor<B, C>& g,
-                                           int first = StageThreads<B, C>::first) {
+__device__ __forceinline__ void stage_load(const EventView& ev, int s, int n, int b,
+                                           StagedSensor<B, C>& g, int first) {
     g.s = -1;
     const int t = (int)threadIdx.x - first;
-    if (s < 0 || t < 0) return;
-    const int n = uni(sh.s_num[s]);
-    if (n > C) return;
-    const int b = uni(sh.s_start[s]);
+    if (s < 0 || t < 0 || n > C) return;
     g.s = s;
 #pragma unroll
     for (int r = 0; r < StagedSensor<B, C>::kPer; ++r) {
@@ -1112,6 +1108,14 @@ __device__ __forceinline__ void stage_load(const Shared<B, C>& sh, const EventVi
     }
     g.x_first = n > 0 ? ev.X[b] : 0.f;
     g.x_last = n > 1 ? ev.X[b + n - 1] : 0.f;
+}
+// the same, reading sensor s's size and start (s >= 0)
+template <int B, int C>
+__device__ __forceinline__ void stage_load(const Shared<B, C>& sh, const EventView& ev, int s,
+                                           StagedSensor<B, C>& g,
+                                           int first = StageThreads<B, C>::first) {
+    const int sc = max(s, 0);
+    stage_load<B, C>(ev, s, uni(sh.s_num[sc]), uni(sh.s_start[sc]), g, first);
 }
 template <int B, int C>
 __device__ __forceinline__ void stage_store(Shared<B, C>& sh, const StagedSensor<B, C>& g,
@@ -1691,7 +1695,14 @@ __device__ __forceinline__ void process_event(const SearchArgs& a, int e, Shared
         if (tid == 0) sh.g_redo = 0;
         prof_mark(sh, pc, PROF_STAGE);
         __syncthreads();
+        // what the step's set-up reads, issued together (each uni() waits
+        // for its read): the list length, the sizes of sensors f .. f-5,
+        // and sensor f-6's size and start
         n_cur = uni(sh.n_listed);
+        int s_n[kStepSensors];
+#pragma unroll
+        for (int d = 0; d < kStepSensors; ++d) s_n[d] = uni(sh.s_num[max(f - d, 0)]);
+        const int n6 = uni(sh.s_num[max(f - 6, 0)]), b6 = uni(sh.s_start[max(f - 6, 0)]);
         // sensor f-6 for the next step, stored at the top of the next step
         // into the slot of sensor f, or during this step into that of f+1
         // (seven slots): by waves that neither forward nor compute windows
@@ -1701,18 +1712,19 @@ __device__ __forceinline__ void process_event(const SearchArgs& a, int e, Shared
         int stage_first = 0;  // > 0: the staging threads start there and store before the forwarding
         if (kStageInForward<B, C>) {
             if (EarlySeed && f - 6 >= 0 && n_cur <= B - kWave) {
-                const int g2 = f - 3, g1 = f - 1;
-                const int nw2 = g2 >= 4 && g1 >= 4 ? n_w2_items(sh, g2) : 0;
-                const int items = (nw2 + kWave - 1) / kWave * kWave + n_w1_items(sh, g1);
+                // the step's window items: h2 windows of f-3, h1 windows of f-1
+                const int nw2 = f - 3 >= 4 && f - 1 >= 4 ? s_n[3] : 0;
+                const int items = (nw2 + kWave - 1) / kWave * kWave + (f - 1 >= 4 ? s_n[1] : 0);
                 const int lo = kWave + (n_cur + kWave - 1) / kWave * kWave;
                 const int hi = B - (items + kWave - 1) / kWave * kWave;
                 // at least one thread: thread `first` writes the slot's
                 // bucket scales (and, for an empty sensor, its starts)
-                if (hi - lo >= max(uni(sh.s_num[f - 6]), 1)) stage_first = lo;
+                if (hi - lo >= max(n6, 1)) stage_first = lo;
             }
-            stage_load<B, C>(sh, ev, f - 6, staged, stage_first > 0 ? stage_first : StageThreads<B, C>::first);
+            stage_load<B, C>(ev, f - 6, n6, b6, staged,
+                             stage_first > 0 ? stage_first : StageThreads<B, C>::first);
         } else if (kAhead) {
-            stage_load<B, C>(sh, ev, f - 6, staged);
+            stage_load<B, C>(ev, f - 6, n6, b6, staged, StageThreads<B, C>::first);
         }
         prof_mark(sh, pc, PROF_STAGE_WAIT);
         prof_add(sh, pc, CNT_FWD_ENTRIES, (unsigned long long)n_cur);
@@ -1722,7 +1734,7 @@ __device__ __forceinline__ void process_event(const SearchArgs& a, int e, Shared
         // reads (a short-circuit chain here made six LDS round trips a step)
         bool lds_step = true;
 #pragma unroll
-        for (int d = 0; d < kStepSensors; ++d) lds_step &= uni(sh.s_num[max(f - d, 0)]) <= C;
+        for (int d = 0; d < kStepSensors; ++d) lds_step &= s_n[d] <= C;
         if (lds_step)
             run_step<true, B, C, EarlySeed>(sh, ev, used, tracks, weak, degenerate, f, st, staged, stage_first, pc);
         else
