@@ -6,14 +6,16 @@
 // Each step is forwarding, then the h0 gather, then seeding, spread over
 // the workgroup; the candidate windows of the next step, which do not
 // depend on the used flags, are computed beside the forwarding by the
-// threads it leaves idle.
+// threads it leaves idle, and in the batch kernel wave 0 gathers the
+// step's first seeding batch beside it too (gathered again if the
+// forwarding marks a hit of the gather sensor).
 //
 // Data placement.  Step f reads hits of sensors f (h0), f-2 (h1), f-4 (h2
 // and the forwarding target) and prepares sensors f-1, f-3 and f-5 for the
 // next step, so the hit coordinates of sensors f .. f-5 sit in a ring of six
 // LDS slots; every step stages one new sensor (512 x 256 and 1024-thread
-// workgroups keep a seventh slot, filled with the next step's sensor while
-// wave 0 gathers alone; see StageThreads).  A sensor with more hits
+// workgroups keep a seventh slot, filled with the next step's sensor during
+// the step; see StageThreads).  A sensor with more hits
 // than a slot holds (C, a template parameter the autotuner picks) is read
 // from HBM instead.  Candidate windows live in small LDS double buffers
 // (HBM for oversized sensors), the used flags in an LDS bitmask, and the
@@ -177,11 +179,14 @@ __device__ __forceinline__ void wave_min_max(float& mn, float& mx) {
 
 // Threads that stage sensors: all but wave 0 for a 512-thread workgroup
 // whose other waves cover a slot with as many hits per thread, and for a
-// 1024-thread one.  Those waves store the next step's sensor f-6 while
-// wave 0 gathers and sets up the seeding batch alone, into a seventh ring
-// slot (the step reads six sensors, f .. f-5, and the slot of f+1 = f-6
-// mod 7 is free then).  Other configurations stage sensor f-5 with all
-// threads at the top of step f, into the slot of f+1 (six slots).
+// 1024-thread one.  Those waves store the next step's sensor f-6 during
+// step f into a seventh ring slot (the step reads six sensors, f .. f-5,
+// and the slot of f+1 = f-6 mod 7 is free then): at 512 threads from
+// registers loaded at the top of the step (by the waves the step leaves
+// idle, or else by these in the forwarding's last tile), at 1024 threads
+// straight from HBM after the forwarding.  Other configurations stage
+// sensor f-5 with all threads at the top of step f, into the slot of f+1
+// (six slots).
 template <int B, int C>
 struct StageThreads {
     static constexpr int first =
@@ -1599,7 +1604,7 @@ __device__ __forceinline__ void process_event(const SearchArgs& a, int e, Shared
             atomicOr(&sh.flags, CLF_FLAG_DEGENERATE_Z);
     }
     // the ring starts with sensors S-1 .. S-5 (S-6 too when the staging runs
-    // during the gather, one step earlier)
+    // during the step, one step earlier)
     constexpr int kEarly = StageThreads<B, C>::during_gather ? 1 : 0;
     for (int s = S - 1; s >= S - 5 - kEarly && s >= 0; --s) stage_sensor<B, C>(sh, ev, s);
     clear_empty_marks<B, C>(sh, 0);
