@@ -214,7 +214,10 @@ def main(argv=None):
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-batch", type=int, default=256)
     ap.add_argument("--serve-wait-us", type=int, default=100)
-    ap.add_argument("--serve-threads", type=int, default=8)
+    # submitter threads of the replay: with 8 the server lock was the
+    # bottleneck (about 5 us of each submit spent waiting for it); 4 gave
+    # the lowest p99 at 75 % and 100 % of the batched peak on one box
+    ap.add_argument("--serve-threads", type=int, default=4)
     ap.add_argument("--profile-phases", action="store_true",
                     help="only print the per-phase time breakdown of one plain and one batched run")
     a = ap.parse_args(argv)
