@@ -82,7 +82,7 @@ hipError_t Resident::ensure(int32_t n, size_t blob_bytes, int64_t hits, int64_t 
     chk(blobs.ensure(blob_bytes));
     chk(desc.ensure(sizeof(EventDesc) * (size_t)n));
     chk(order.ensure(sizeof(int32_t) * (size_t)n));
-    chk(queue.ensure(sizeof(int32_t) * ((size_t)n + 1)));
+    chk(queue.ensure(sizeof(int32_t) * queue_words(n)));
     chk(cand.ensure(sizeof(int4) * (size_t)hits));
     chk(used.ensure((size_t)hits));
     chk(tracks.ensure(sizeof(clf_track) * (size_t)cap));
@@ -116,7 +116,7 @@ SearchArgs Resident::args(int32_t n) const {
     a.order = order.as<int32_t>();
     a.first_item = 0;
     a.n_items = n;
-    a.queue_head = queue.as<int32_t>();
+    a.queue_head = queue_counters();
     a.grain = 1;
     a.cand = cand.as<int4>();
     a.used = used.as<uint8_t>();
@@ -127,11 +127,17 @@ SearchArgs Resident::args(int32_t n) const {
     a.slot_records = 0;
     a.slot_base = 0;
     a.results = results.as<EventResult>();
-    // prefix[e]: first packed record of event e; prefix[n]: the allocator,
-    // which ends as the number of packed records
+    // prefix[e]: first packed record of event e; prefix[n]: the number of
+    // packed records, stored by the run's last workgroup from the allocator
+    // (queue header words 0-1); `done` is header word 2
     a.packed = packed.as<clf_track>();
     a.track_base = prefix.as<int64_t>();
-    a.track_alloc = reinterpret_cast<unsigned long long*>(prefix.as<int64_t>() + n);
+    a.track_alloc = reinterpret_cast<unsigned long long*>(queue.as<int32_t>());
+    a.done = reinterpret_cast<unsigned int*>(queue.as<int32_t>() + 2);
+    a.done_total = 0;  // the workgroups of the run, set where it is launched
+    a.reset_queue = queue_counters();
+    a.reset_n = n + 1;
+    a.track_total = prefix.as<int64_t>() + n;
     a.fill_only = 0;
     a.prof = nullptr;
     return a;
@@ -169,13 +175,14 @@ int64_t slot_records(const Resident& r, int64_t slots) {
 
 clf_status enqueue_search(const Resident& r, int32_t n, clf_mode mode,
                           const clf_launch_config& cfg, int n_cu, hipStream_t st, hipEvent_t t0,
-                          hipEvent_t t1, hipStream_t side, hipEvent_t fork, hipEvent_t join) {
+                          hipEvent_t t1, hipStream_t side, hipEvent_t fork, hipEvent_t join,
+                          bool clear) {
     if (n == 0) return CLF_OK;
     SearchArgs a = r.args(n);
     a.grain = std::max(1, cfg.grain);
-    // queue counters, one per launch (n+1 so plain mode has one per event),
-    // and the packed-output allocator
-    HIP_TRY(launch_zero_i32(r.queue.as<int32_t>(), n + 1, st, a.track_alloc));
+    // the allocator, the run's count of finished workgroups and the queue
+    // counters (one per launch, n + 1 so plain mode has one per event)
+    if (clear) HIP_TRY(launch_zero_i32(r.queue.as<int32_t>(), (int)Resident::queue_words(n), st));
     if (t0) HIP_TRY(hipEventRecord(t0, st));
     const int heavy = side && fork && join && cfg.heavy_events > 0 ? std::min(cfg.heavy_events, n) : 0;
     if (mode == CLF_MODE_BATCHED && heavy > 0) {
@@ -187,10 +194,11 @@ clf_status enqueue_search(const Resident& r, int32_t n, clf_mode mode,
         // work slots: the heavy launch's first, then the main launch's
         const int grid = heavy < n ? resolved_grid(cfg, n_cu, n - heavy) : 0;
         a.slot_records = slot_records(r, (int64_t)heavy + grid);
+        a.done_total = heavy + grid;  // both launches end the run together
         SearchArgs h = a;
         h.first_item = 0;
         h.n_items = heavy;
-        h.queue_head = r.queue.as<int32_t>() + 1;
+        h.queue_head = r.queue_counters() + 1;
         HIP_TRY(launch_search(h, cfg.heavy_block, cfg.lds_hits, heavy, side));
         HIP_TRY(hipEventRecord(join, side));
         if (heavy < n) {
@@ -203,16 +211,19 @@ clf_status enqueue_search(const Resident& r, int32_t n, clf_mode mode,
     } else if (mode == CLF_MODE_BATCHED) {
         const int grid = resolved_grid(cfg, n_cu, n);
         a.slot_records = slot_records(r, grid);
+        a.done_total = grid;
         HIP_TRY(launch_search(a, cfg.block_size, cfg.lds_hits, grid, st));
     } else {
         // plain: event i alone, in its own launch of one workgroup; the
         // launches run one after another on the stream and share one slot
+        // (the last one ends the run)
         a.order = nullptr;
         a.n_items = 1;
         a.slot_records = slot_records(r, 1);
+        a.done_total = n;
         for (int i = 0; i < n; ++i) {
             a.first_item = i;
-            a.queue_head = r.queue.as<int32_t>() + 1 + i;
+            a.queue_head = r.queue_counters() + 1 + i;
             HIP_TRY(launch_search(a, cfg.block_size, cfg.lds_hits, 1, st));
         }
     }
@@ -269,12 +280,14 @@ namespace {
 
 hipError_t current_device(clf_context* ctx) { return hipSetDevice(ctx->device); }
 
-// The kernels of one run, enqueued on the context stream.
-clf_status enqueue_run(clf_context* ctx, clf_batch* b, clf_mode mode, bool time_search) {
+// The kernels of one run, enqueued on the context stream (clear: with the
+// clear of the queue words before them).
+clf_status enqueue_run(clf_context* ctx, clf_batch* b, clf_mode mode, bool time_search,
+                       bool clear = true) {
     return enqueue_search(b->dev, b->n, mode, resolve_launch(ctx->cfg, b->total_hits, b->n),
                           ctx->n_cu, ctx->stream,
                           time_search ? ctx->ev2 : nullptr, time_search ? ctx->ev3 : nullptr,
-                          ctx->side, ctx->fork, ctx->join);
+                          ctx->side, ctx->fork, ctx->join, clear);
 }
 
 bool same_cfg(const clf_launch_config& x, const clf_launch_config& y) {
@@ -293,9 +306,12 @@ clf_status run(clf_context* ctx, clf_batch* b, clf_mode mode) {
             b->graph = nullptr;
         }
         if (!b->graph) {
+            // the graph holds the search launch(es) alone: every run ends
+            // with its queue words at zero, so only the first replay needs
+            // them cleared, here, before it
             hipGraph_t g = nullptr;
             HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            clf_status s = enqueue_run(ctx, b, mode, false);
+            clf_status s = enqueue_run(ctx, b, mode, false, false);
             hipError_t e = hipStreamEndCapture(ctx->stream, &g);
             if (s != CLF_OK) {
                 if (g) (void)hipGraphDestroy(g);
@@ -306,6 +322,8 @@ clf_status run(clf_context* ctx, clf_batch* b, clf_mode mode) {
             (void)hipGraphDestroy(g);
             if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
             b->graph_cfg = lc;
+            HIP_TRY(launch_zero_i32(b->dev.queue.as<int32_t>(), (int)Resident::queue_words(b->n),
+                                    ctx->stream));
         }
         HIP_TRY(hipGraphLaunch(b->graph, ctx->stream));
         return CLF_OK;
@@ -723,18 +741,20 @@ clf_status clf_batch_profile(clf_context* ctx, clf_batch* b, clf_mode mode, uint
     SearchArgs a = b->dev.args(n);
     a.prof = prof.as<unsigned long long>();
     const clf_launch_config lc = resolve_launch(ctx->cfg, b->total_hits, n);
-    HIP_TRY(launch_zero_i32(b->dev.queue.as<int32_t>(), n + 1, st, a.track_alloc));
+    HIP_TRY(launch_zero_i32(b->dev.queue.as<int32_t>(), (int)Resident::queue_words(n), st));
     if (mode == CLF_MODE_BATCHED) {
         const int grid = resolved_grid(lc, ctx->n_cu, n);
         a.slot_records = slot_records(b->dev, grid);
+        a.done_total = grid;
         HIP_TRY(launch_search(a, lc.block_size, lc.lds_hits, grid, st));
     } else {
         a.order = nullptr;
         a.n_items = 1;
         a.slot_records = slot_records(b->dev, 1);
+        a.done_total = n;
         for (int i = 0; i < n; ++i) {
             a.first_item = i;
-            a.queue_head = b->dev.queue.as<int32_t>() + 1 + i;
+            a.queue_head = b->dev.queue_counters() + 1 + i;
             HIP_TRY(launch_search(a, lc.block_size, lc.lds_hits, 1, st));
         }
     }
@@ -791,11 +811,12 @@ clf_status fill_windows(clf_context* ctx, int32_t n, const uint8_t* const* event
     } guard{b};
     if (n == 0) return CLF_OK;
     hipStream_t st = ctx->stream;
-    HIP_TRY(launch_zero_i32(b->dev.queue.as<int32_t>(), n + 1, st));
+    HIP_TRY(launch_zero_i32(b->dev.queue.as<int32_t>(), (int)Resident::queue_words(n), st));
     SearchArgs a = b->dev.args(n);
     a.fill_only = mode;
     const clf_launch_config lc = resolve_launch(ctx->cfg, b->total_hits, n);
-    HIP_TRY(launch_search(a, lc.block_size, lc.lds_hits, resolved_grid(lc, ctx->n_cu, n), st));
+    a.done_total = resolved_grid(lc, ctx->n_cu, n);
+    HIP_TRY(launch_search(a, lc.block_size, lc.lds_hits, a.done_total, st));
     std::vector<int4> cand((size_t)std::max<int64_t>(b->total_hits, 1));
     std::vector<EventResult> res(n);
     if (b->total_hits > 0)
@@ -928,7 +949,7 @@ clf_status clf_search_noalloc(clf_context* ctx, clf_mode mode, int32_t n,
     };
     const size_t o_blobs = carve(off), o_desc = carve(sizeof(EventDesc) * (size_t)n),
                  o_order = carve(sizeof(int32_t) * (size_t)n),
-                 o_queue = carve(sizeof(int32_t) * ((size_t)n + 1)),
+                 o_queue = carve(sizeof(int32_t) * Resident::queue_words(n)),
                  o_cand = carve(sizeof(int4) * (size_t)hits), o_used = carve((size_t)hits),
                  o_tracks = carve(sizeof(clf_track) * (size_t)cap),
                  o_ttf_a = carve(sizeof(TtfEntry) * (size_t)cap),

@@ -59,7 +59,7 @@ struct SearchArgs {
     const int32_t* order;  // dequeue order of events (nullptr: identity)
     int32_t first_item;    // items [first_item, first_item + n_items) of order
     int32_t n_items;
-    int32_t* queue_head;   // work-queue counter, zero before the launch
+    int32_t* queue_head;   // work-queue counter, zero before the run
     int32_t grain;         // consecutive queue items taken per dequeue (>= 1)
     int4* cand;            // per hit, 16 B: stage (a) output of the fill-only
                            // check; otherwise W1Rec[N] then W2Rec[N] of
@@ -79,11 +79,22 @@ struct SearchArgs {
     EventResult* results;  // per event
     // Packed output (nullptr: none, the caller copies from `tracks`): at the
     // end of event e its workgroup reserves n_tracks records at
-    // *track_alloc (zero before the launch), records the first one in
+    // *track_alloc (zero before the run), records the first one in
     // track_base[e] and copies the event's tracks there.
     clf_track* packed;
     int64_t* track_base;
     unsigned long long* track_alloc;
+    // End of a run (done nullptr: none).  `done` counts the workgroups that
+    // have left, over every launch of the run; the one that makes it
+    // done_total stores the allocator's final value in *track_total, then
+    // clears the allocator, the run's queue counters (reset_queue[0,
+    // reset_n)) and `done`: the next run finds them at zero with no
+    // clearing launch before it.
+    unsigned int* done;
+    int32_t done_total;
+    int32_t* reset_queue;
+    int32_t reset_n;
+    int64_t* track_total;
     int32_t fill_only;     // stop after the candidate windows (stage (a) check)
     // Optional per-event phase profile (nullptr: off): kProfSlots counters per
     // event, see the PROF_* indices in clf_kernels.hip.

@@ -178,6 +178,12 @@ struct Resident {
     void release();
     void keep_larger(Resident& o);
     SearchArgs args(int32_t n) const;
+    // The queue words: a 16-byte header (the packed-output allocator, a
+    // uint64, then the run's count of finished workgroups) and n + 1 queue
+    // counters (plain mode takes one per event, counter 1 + i).
+    static constexpr int kQueueHeaderWords = 4;
+    static size_t queue_words(int32_t n) { return (size_t)kQueueHeaderWords + (size_t)n + 1; }
+    int32_t* queue_counters() const { return queue.as<int32_t>() + kQueueHeaderWords; }
 };
 
 // HIP device ordinal of a context.
@@ -194,11 +200,6 @@ int resolved_grid(const clf_launch_config& cfg, int n_cu, int n_items);
 constexpr int64_t kAutoLargeEventHits = 4096;
 clf_launch_config resolve_launch(const clf_launch_config& cfg, int64_t hits, int32_t n);
 
-// Enqueues one search of n resident events on `stream`: the clear of the
-// queue counters and the output allocator, then the search kernel(s), which
-// pack each event's tracks (track_base[e], the `prefix` buffer, holds where
-// they start).  cfg must be resolved (resolve_launch).  t0/t1 (nullable)
-// are recorded right around the search kernel(s).
 // Records per work slot when `slots` workgroups each take one slot of the
 // batch's largest slice and the slots fit where its per-event slices lie
 // (every workgroup then reuses its slot, event after event, so the slices'
@@ -206,11 +207,19 @@ clf_launch_config resolve_launch(const clf_launch_config& cfg, int64_t hits, int
 // event).
 int64_t slot_records(const Resident& r, int64_t slots);
 
-// side/fork/join (nullable): the stream and events of the heavy-event
-// launch (cfg.heavy_events); without them everything runs in one launch.
+// Enqueues one search of n resident events on `stream`: with `clear`, the
+// clear of the queue words (a run leaves them at zero, but fresh buffers
+// hold anything), then the search kernel(s), which pack each event's
+// tracks (track_base[e], the `prefix` buffer, holds where they start; the
+// run's last workgroup writes the total to prefix[n]).  cfg must be
+// resolved (resolve_launch).  t0/t1 (nullable) are recorded right around
+// the search kernel(s).  side/fork/join (nullable): the stream and events
+// of the heavy-event launch (cfg.heavy_events); without them everything
+// runs in one launch.
 clf_status enqueue_search(const Resident& r, int32_t n, clf_mode mode,
                           const clf_launch_config& cfg, int n_cu, hipStream_t stream,
                           hipEvent_t t0, hipEvent_t t1, hipStream_t side = nullptr,
-                          hipEvent_t fork = nullptr, hipEvent_t join = nullptr);
+                          hipEvent_t fork = nullptr, hipEvent_t join = nullptr,
+                          bool clear = true);
 
 }  // namespace clf

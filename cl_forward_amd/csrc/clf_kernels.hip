@@ -1836,6 +1836,33 @@ __device__ __forceinline__ void process_event(const SearchArgs& a, int e, Shared
     }
 }
 
+// The end of a run, by every workgroup after its last dequeue: the one
+// that finishes last (the count of finished workgroups reaches done_total
+// over all of the run's launches) stores the number of packed records and
+// clears the allocator, the run's queue counters and the count, so that
+// the next run, or the next replay of a graph holding this launch alone,
+// starts from zero.  Each workgroup's allocations and dequeues precede its
+// count (fenced), so the last one sees the allocator's final value.
+template <int B, int C>
+__device__ __forceinline__ void run_end(const SearchArgs& a, Shared<B, C>& sh) {
+    if (!a.done) return;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        sh.item = atomicAdd(a.done, 1u) + 1u == (unsigned int)a.done_total ? 1 : 0;
+    }
+    __syncthreads();
+    if (uni(sh.item) == 0) return;
+    __threadfence();
+    for (int i = threadIdx.x; i < a.reset_n; i += B) a.reset_queue[i] = 0;
+    if (threadIdx.x == 0) {
+        if (a.track_alloc) {
+            *a.track_total = (int64_t)atomicAdd(a.track_alloc, 0ull);
+            *a.track_alloc = 0ull;
+        }
+        *a.done = 0u;
+    }
+}
+
 // Waves per SIMD the register allocation is held to (512 registers per
 // lane / waves).  An event's search time is set by its waves' latency
 // chains, so a CU runs more events at once, and a heavy event finishes
@@ -1868,6 +1895,7 @@ void search_kernel(SearchArgs a) {
             __syncthreads();
         }
     }
+    run_end(a, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -2192,9 +2220,10 @@ void serve_kernel(ServeArgs a) {
     }
 }
 
-// Writes zeros: the queue counters of a launch, and its packed-output
-// allocator (a kernel, not a memset, so that the sequence can be captured
-// into a graph and replayed).
+// Writes zeros: a batch's queue words (the packed-output allocator, the
+// count of finished workgroups and the queue counters) before an eager run
+// or a graph's first replay, and other control words (a kernel, not a
+// memset, so that a sequence holding it can be captured into a graph).
 __global__ void zero_i32_kernel(int32_t* p, int n, unsigned long long* alloc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;

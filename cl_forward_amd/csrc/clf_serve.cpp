@@ -696,6 +696,11 @@ ServeArgs serve_args(clf_server* s) {
     b.packed = nullptr;  // serve_one copies the tracks to the request itself
     b.track_base = nullptr;
     b.track_alloc = nullptr;
+    b.done = nullptr;  // no run end: the serving kernel never leaves its queue
+    b.done_total = 0;
+    b.reset_queue = nullptr;
+    b.reset_n = 0;
+    b.track_total = nullptr;
     b.fill_only = 0;
     b.prof = nullptr;
     void* p = nullptr;

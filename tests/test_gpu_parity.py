@@ -186,6 +186,48 @@ def test_determinism_and_graph_replays(gpu_ctx, plain51_blobs):
         assert plain == first
 
 
+def test_every_run_leaves_its_queue_words_clear(oracle_mod, config_blobs):
+    """A graph holds the search launch alone: each run's last workgroup
+    stores the packed total and clears the allocator, the finished-workgroup
+    count and the queue counters for the next run.  Replays of one batch's
+    graph between every other kind of run on the same buffers (plain
+    launches, a phase profile, uncaptured runs, the timing runs, the heavy
+    lane, other launch shapes, a coarser queue grain) give the oracle's
+    bytes each time, through the packed total and the offsets."""
+    from cl_forward_amd.search import Context
+
+    want = [oracle_mod.search_event(x)[0] for x in config_blobs]
+    with Context(0) as ctx:
+        ctx.set_config(block_size=512, lds_hits=256, use_graph=1)
+        with ctx.batch(config_blobs) as b:
+            def check(tag, mode=MODE_BATCHED):
+                b.run(mode)
+                outs, _ = b.results()
+                assert outs == want, tag
+
+            check("first replay")
+            check("second replay")
+            check("plain launches", MODE_PLAIN)
+            check("replay after plain launches")
+            b.profile(MODE_BATCHED)
+            check("replay after a batched profile")
+            b.profile(MODE_PLAIN)
+            check("replay after a plain profile")
+            ctx.set_config(use_graph=0)  # same launch shape: the graph stays
+            check("uncaptured run")
+            ctx.set_config(use_graph=1)
+            check("replay after an uncaptured run")
+            b.time(MODE_BATCHED, warmup=1, reps=2)
+            check("replay after the timing runs")
+            for kw in (dict(heavy_events=3, heavy_block=512), dict(heavy_events=0),
+                       dict(block_size=1024, lds_hits=512), dict(grain=2),
+                       dict(block_size=512, lds_hits=256, grain=1)):
+                ctx.set_config(**kw)
+                _say(f"search: queue words after a change to {kw}")
+                check(f"new graph {kw}")
+                check(f"its replay {kw}")
+
+
 def test_stress_events(gpu_ctx, oracle_mod):
     _say("search: 6 high-occupancy events (sensors above 128 hits)")
     blobs = [generator.stress_event(i).pack() for i in range(6)]
