@@ -4,6 +4,7 @@
 #include <atomic>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -255,6 +256,10 @@ struct clf_context {
     std::vector<EventDesc> na_desc;
     std::vector<int32_t> na_order;
     std::vector<double> na_cost;
+    // search_each: one event per chunk of the read-back, created on first
+    // use and kept (each_mu holds them for a call)
+    std::mutex each_mu;
+    std::vector<hipEvent_t> each_ev;
 };
 
 struct clf_batch {
@@ -415,6 +420,7 @@ void clf_context_destroy(clf_context* ctx) {
     if (ctx->join) (void)hipEventDestroy(ctx->join);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    for (hipEvent_t e : ctx->each_ev) (void)hipEventDestroy(e);
     ctx->spare.release();
     ctx->spare_blobs.release();
     ctx->spare_results.release();
@@ -1031,3 +1037,125 @@ clf_status clf_search(clf_context* ctx, clf_mode mode, int32_t n, const uint8_t*
 }
 
 }  // extern "C"
+
+namespace clf {
+
+clf_status search_each(clf_context* ctx, clf_mode mode, int32_t n, const uint8_t* const* events,
+                       const size_t* sizes,
+                       const std::function<void(int, const uint8_t*, size_t)>& emit,
+                       clf_batch** keep) {
+    if (!ctx || n < 0 || (n > 0 && (!events || !sizes)))
+        return fail(CLF_E_INVALID_ARGUMENT, "null argument or negative count");
+    if (keep) *keep = nullptr;
+    if (mode != CLF_MODE_PLAIN && mode != CLF_MODE_BATCHED)
+        return fail(CLF_E_INVALID_ARGUMENT, "unknown mode");
+    clf_batch* b = nullptr;
+    clf_status s = batch_create(ctx, n, events, sizes, &b, false);
+    if (s != CLF_OK) return s;
+    struct Guard {
+        clf_batch*& b;
+        clf_batch** keep;
+        ~Guard() {
+            if (keep && b) *keep = b;
+            else clf_batch_destroy(b);
+        }
+    } guard{b, keep};
+    s = enqueue_run(ctx, b, mode, false);
+    if (s != CLF_OK || n == 0) return s;
+    std::lock_guard<std::mutex> lk(ctx->each_mu);
+    hipStream_t st = ctx->stream;
+    // records of the packed output per read-back chunk (2 MiB)
+    constexpr int64_t kChunkRecs = (int64_t(2) << 20) / (int64_t)sizeof(clf_track);
+    const size_t est_bytes = sizeof(clf_track) * (size_t)b->total_cap;
+    const int threads = est_bytes >= (size_t(4) << 20) ? 8 : 0;
+
+    // The hand-over: events in the order of their records; event j waits
+    // for the chunk holding its last record.  `ready` is -1 until the order
+    // is known, then the number of chunks in host memory; on an error the
+    // workers leave without handing anything over.
+    std::vector<int> by_off(n);
+    std::vector<int> need(n);
+    std::atomic<int> ready{-1}, next{0};
+    std::atomic<bool> failed{false};
+    const int64_t* pre = nullptr;
+    const clf_event_stats* stv = nullptr;
+    const uint8_t* host = nullptr;
+    auto hand_over = [&]() {
+        // the order (by_off, need) is read only once it is published
+        while (!failed.load(std::memory_order_acquire) && ready.load(std::memory_order_acquire) < 0)
+            std::this_thread::yield();
+        if (failed.load(std::memory_order_acquire)) return;
+        for (int j; (j = next.fetch_add(1)) < n;) {
+            const int i = by_off[j];
+            while (!failed.load(std::memory_order_acquire) && ready.load(std::memory_order_acquire) <= need[i])
+                std::this_thread::yield();
+            if (failed.load(std::memory_order_acquire)) return;
+            emit(i, host + sizeof(clf_track) * (size_t)pre[i],
+                 sizeof(clf_track) * (size_t)std::max(stv[i].n_tracks, 0));
+        }
+    };
+    // the workers start while the search runs
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t) pool.emplace_back(hand_over);
+    struct Join {
+        std::vector<std::thread>& pool;
+        std::atomic<bool>& failed;
+        std::atomic<int>& ready;
+        ~Join() {
+            if (ready.load() < 0) failed.store(true, std::memory_order_release);
+            for (std::thread& t : pool) t.join();
+        }
+    } join{pool, failed, ready};
+    auto stop = [&](clf_status e) {
+        failed.store(true, std::memory_order_release);
+        return e;
+    };
+
+    hipError_t e = hipMemcpyAsync(b->h_results.p, b->dev.results.p, sizeof(EventResult) * (size_t)n,
+                                  hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->h_prefix.p, b->dev.prefix.p, sizeof(int64_t) * ((size_t)n + 1),
+                           hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return stop(hip_fail(e, "search read-back"));
+    pre = b->h_prefix.as<int64_t>();
+    stv = b->h_results.as<clf_event_stats>();
+    const int64_t total = pre[n];
+    if (total < 0 || total > b->total_cap) return stop(fail(CLF_E_DEVICE, "inconsistent track count"));
+    for (int i = 0; i < n; ++i)
+        if (pre[i] < 0 || stv[i].n_tracks < 0 || pre[i] + stv[i].n_tracks > total)
+            return stop(fail(CLF_E_DEVICE, "inconsistent track offsets"));
+    e = b->h_packed.ensure(sizeof(clf_track) * (size_t)std::max<int64_t>(total, 1));
+    if (e != hipSuccess) return stop(hip_fail(e, "pinned read-back buffer"));
+    host = b->h_packed.as<uint8_t>();
+    std::iota(by_off.begin(), by_off.end(), 0);
+    std::sort(by_off.begin(), by_off.end(), [&](int x, int y) { return pre[x] < pre[y]; });
+    for (int i = 0; i < n; ++i)
+        need[i] = stv[i].n_tracks > 0 ? (int)((pre[i] + stv[i].n_tracks - 1) / kChunkRecs) : -1;
+    const int chunks = (int)((total + kChunkRecs - 1) / kChunkRecs);
+    while ((int)ctx->each_ev.size() < chunks) {
+        hipEvent_t ev = nullptr;
+        e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e != hipSuccess) return stop(hip_fail(e, "hipEventCreateWithFlags"));
+        ctx->each_ev.push_back(ev);
+    }
+    const uint8_t* dev = b->dev.packed.as<uint8_t>();
+    for (int k = 0; k < chunks && e == hipSuccess; ++k) {
+        const int64_t r0 = k * kChunkRecs, r1 = std::min(total, r0 + kChunkRecs);
+        e = hipMemcpyAsync(b->h_packed.as<uint8_t>() + sizeof(clf_track) * (size_t)r0,
+                           dev + sizeof(clf_track) * (size_t)r0, sizeof(clf_track) * (size_t)(r1 - r0),
+                           hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipEventRecord(ctx->each_ev[k], st);
+    }
+    if (e != hipSuccess) return stop(hip_fail(e, "chunked read-back"));
+    ready.store(0, std::memory_order_release);
+    for (int k = 0; k < chunks; ++k) {
+        e = hipEventSynchronize(ctx->each_ev[k]);
+        if (e != hipSuccess) return stop(hip_fail(e, "chunked read-back"));
+        ready.store(k + 1, std::memory_order_release);
+    }
+    if (threads == 0) hand_over();
+    return CLF_OK;
+}
+
+}  // namespace clf

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <thread>
 #include <vector>
@@ -221,5 +222,17 @@ clf_status enqueue_search(const Resident& r, int32_t n, clf_mode mode,
                           hipEvent_t t0, hipEvent_t t1, hipStream_t side = nullptr,
                           hipEvent_t fork = nullptr, hipEvent_t join = nullptr,
                           bool clear = true);
+
+// The one-call search of clf_search with the tracks handed over event by
+// event: emit(i, records, bytes) runs for every event i exactly once (bytes
+// 0 for an event without tracks) before this returns, on up to eight host
+// threads, as soon as the part of the packed output holding its records is
+// in host memory: the output comes back in 2 MiB chunks, and the events
+// whose records have arrived are handed over while the next chunks are in
+// flight.  keep (nullable): receives the batch, for the caller to destroy.
+clf_status search_each(clf_context* ctx, clf_mode mode, int32_t n, const uint8_t* const* events,
+                       const size_t* sizes,
+                       const std::function<void(int, const uint8_t*, size_t)>& emit,
+                       clf_batch** keep = nullptr);
 
 }  // namespace clf

@@ -1,5 +1,6 @@
 // clf_shims.cpp — the reference's C++ entry points (GpuPixelSearchByTriplet.h,
-// KernelInvoker.h) as thin shims over the C ABI of include/clf.h.
+// KernelInvoker.h) as thin shims over the library (include/clf.h, and the
+// event-by-event read-back of clf_host.h).
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -15,6 +16,7 @@
 #include "GpuPixelSearchByTriplet.h"
 #include "KernelInvoker.h"
 #include "clf.h"
+#include "clf_host.h"
 
 namespace {
 
@@ -102,36 +104,20 @@ int invoke_locked(int start, int count, const std::vector<const std::vector<uint
     }
     if (g_verbosity >= 3)
         std::cout << "Invoking kernels on your " << clf_context_device_name(ctx) << std::endl;
-    clf_result res;
+    // each event's records into its output vector (KernelInvoker.cpp:197-206)
+    // as soon as they are in host memory: the read-back comes in chunks and
+    // the copies into the vectors run on several host threads beside it
     clf_batch* batch = nullptr;
-    clf_status s = clf_search(ctx, CLF_MODE_BATCHED, count, ptrs.data(), sizes.data(), &res, &batch);
+    clf_status s = clf::search_each(
+        ctx, CLF_MODE_BATCHED, count, ptrs.data(), sizes.data(),
+        [&](int i, const uint8_t* src, size_t bytes) { output[start + i].assign(src, src + bytes); },
+        g_verbosity >= 3 ? &batch : nullptr);
     if (s != CLF_OK) fatal(s, "clf_search");
     TimingReport timing;
-    if (g_verbosity >= 3 && count > 0) run_timing(ctx, batch, timing);
-    // each event's records into its output vector (KernelInvoker.cpp:197-206),
-    // on several host threads when there is much to copy
-    const int64_t total = count > 0 ? res.track_offset[count] : 0;
-    const int threads = total * (int64_t)sizeof(clf_track) >= (int64_t(4) << 20) ? 8 : 1;
-    const int per = (count + 4 * threads - 1) / (4 * threads);
-    auto scatter = [&](int k) {
-        for (int i = k * per; i < std::min(count, (k + 1) * per); ++i) {
-            const uint8_t* src = res.tracks + res.track_offset[i] * sizeof(clf_track);
-            output[start + i].assign(src, src + (size_t)res.stats[i].n_tracks * sizeof(clf_track));
-        }
-    };
-    const int chunks = per > 0 ? (count + per - 1) / per : 0;
-    if (threads <= 1) {
-        for (int k = 0; k < chunks; ++k) scatter(k);
-    } else {
-        std::atomic<int> next{0};
-        std::vector<std::thread> pool;
-        for (int t = 0; t < threads; ++t)
-            pool.emplace_back([&]() {
-                for (int k; (k = next.fetch_add(1)) < chunks;) scatter(k);
-            });
-        for (std::thread& t : pool) t.join();
+    if (batch) {
+        if (count > 0) run_timing(ctx, batch, timing);
+        clf_batch_destroy(batch);
     }
-    clf_batch_destroy(batch);
     if (g_verbosity >= 3 && count > 0) print_timing(timing);
     return 0;
 }
