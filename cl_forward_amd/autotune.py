@@ -88,7 +88,7 @@ def tune(ctx, batch, reps: int = 5, warmup: int = 1, log=print) -> dict:
     ref = {}
 
     def measure(**cfg):
-        ctx.set_config(use_graph=1, **cfg)
+        ctx.set_config(use_graph=0, **cfg)
         t = batch.time(MODE_BATCHED, warmup=warmup, reps=reps)
         batch.run(MODE_BATCHED)
         outs, _ = batch.results()
@@ -114,7 +114,7 @@ def tune(ctx, batch, reps: int = 5, warmup: int = 1, log=print) -> dict:
                 lds_hits=base["lds_hits"], grain=1, heavy_events=k, heavy_block=hb)
     best = pick_best(table)
     return {"best": {"block_size": best["block_size"], "grid_size": best["grid_size"],
-                     "lds_hits": best["lds_hits"], "grain": best.get("grain", 1), "use_graph": 1,
+                     "lds_hits": best["lds_hits"], "grain": best.get("grain", 1), "use_graph": 0,
                      "heavy_events": best["heavy_events"], "heavy_block": best["heavy_block"]},
             "table": table}
 

@@ -239,7 +239,8 @@ struct clf_context {
     char name[256] = {0};
     hipStream_t stream = nullptr;
     int n_cu = 0;
-    clf_launch_config cfg = {0, 0, 1, 1, 0, 0, 512};  // automatic launch shape
+    // automatic launch shape, direct launches (use_graph 0)
+    clf_launch_config cfg = {0, 0, 1, 0, 0, 0, 512};
     hipStream_t side = nullptr;  // the heavy-event launch of the batched mode
     hipEvent_t fork = nullptr, join = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
@@ -275,6 +276,9 @@ struct clf_batch {
     hipGraphExec_t graph = nullptr;
     clf_launch_config graph_cfg = {};
     bool results_valid = false;
+    // the queue words are at zero for the next run: a run has been enqueued
+    // (each leaves them so), or they were cleared; fresh buffers hold anything
+    bool words_clear = false;
 };
 
 namespace clf {
@@ -312,8 +316,8 @@ clf_status run(clf_context* ctx, clf_batch* b, clf_mode mode) {
         }
         if (!b->graph) {
             // the graph holds the search launch(es) alone: every run ends
-            // with its queue words at zero, so only the first replay needs
-            // them cleared, here, before it
+            // with its queue words at zero, so only a batch's first run
+            // needs them cleared, before it
             hipGraph_t g = nullptr;
             HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
             clf_status s = enqueue_run(ctx, b, mode, false, false);
@@ -327,13 +331,18 @@ clf_status run(clf_context* ctx, clf_batch* b, clf_mode mode) {
             (void)hipGraphDestroy(g);
             if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
             b->graph_cfg = lc;
+        }
+        if (!b->words_clear)
             HIP_TRY(launch_zero_i32(b->dev.queue.as<int32_t>(), (int)Resident::queue_words(b->n),
                                     ctx->stream));
-        }
         HIP_TRY(hipGraphLaunch(b->graph, ctx->stream));
+        b->words_clear = true;
         return CLF_OK;
     }
-    return enqueue_run(ctx, b, mode, false);
+    // a direct launch: the clear only before the batch's first run
+    clf_status s = enqueue_run(ctx, b, mode, false, !b->words_clear);
+    if (s == CLF_OK) b->words_clear = true;
+    return s;
 }
 
 }  // namespace
