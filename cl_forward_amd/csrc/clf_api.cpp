@@ -717,7 +717,8 @@ clf_status clf_batch_time(clf_context* ctx, clf_batch* b, clf_mode mode, int32_t
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     out->ms_run = ms / (float)reps;
-    // search kernel(s) alone, without the graph, events right around them
+    // search kernel(s) alone, launched directly after a clear of the queue
+    // words, with events right around them
     float total = 0.f;
     for (int i = 0; i < reps; ++i) {
         clf_status s = enqueue_run(ctx, b, mode, true);

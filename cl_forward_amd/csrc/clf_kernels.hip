@@ -2221,9 +2221,9 @@ void serve_kernel(ServeArgs a) {
 }
 
 // Writes zeros: a batch's queue words (the packed-output allocator, the
-// count of finished workgroups and the queue counters) before an eager run
-// or a graph's first replay, and other control words (a kernel, not a
-// memset, so that a sequence holding it can be captured into a graph).
+// count of finished workgroups and the queue counters) before its first
+// run, and other control words (a kernel, not a memset, so that a sequence
+// holding it can be captured into a graph).
 __global__ void zero_i32_kernel(int32_t* p, int n, unsigned long long* alloc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
