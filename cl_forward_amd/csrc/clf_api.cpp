@@ -1062,16 +1062,22 @@ clf_status search_each(clf_context* ctx, clf_mode mode, int32_t n, const uint8_t
     clf_batch* b = nullptr;
     clf_status s = batch_create(ctx, n, events, sizes, &b, false);
     if (s != CLF_OK) return s;
+    // the batch goes to `keep` only when the call succeeds
     struct Guard {
         clf_batch*& b;
         clf_batch** keep;
+        bool ok = false;
         ~Guard() {
-            if (keep && b) *keep = b;
+            if (keep && ok) *keep = b;
             else clf_batch_destroy(b);
         }
     } guard{b, keep};
     s = enqueue_run(ctx, b, mode, false);
-    if (s != CLF_OK || n == 0) return s;
+    if (s != CLF_OK) return s;
+    if (n == 0) {
+        guard.ok = true;
+        return CLF_OK;
+    }
     std::lock_guard<std::mutex> lk(ctx->each_mu);
     hipStream_t st = ctx->stream;
     // records of the packed output per read-back chunk (2 MiB)
@@ -1104,9 +1110,9 @@ clf_status search_each(clf_context* ctx, clf_mode mode, int32_t n, const uint8_t
                  sizeof(clf_track) * (size_t)std::max(stv[i].n_tracks, 0));
         }
     };
-    // the workers start while the search runs
+    // the workers start while the search runs; whatever happens after, they
+    // are joined (told to leave first if the order was never published)
     std::vector<std::thread> pool;
-    for (int t = 0; t < threads; ++t) pool.emplace_back(hand_over);
     struct Join {
         std::vector<std::thread>& pool;
         std::atomic<bool>& failed;
@@ -1116,6 +1122,7 @@ clf_status search_each(clf_context* ctx, clf_mode mode, int32_t n, const uint8_t
             for (std::thread& t : pool) t.join();
         }
     } join{pool, failed, ready};
+    for (int t = 0; t < threads; ++t) pool.emplace_back(hand_over);
     auto stop = [&](clf_status e) {
         failed.store(true, std::memory_order_release);
         return e;
@@ -1165,6 +1172,7 @@ clf_status search_each(clf_context* ctx, clf_mode mode, int32_t n, const uint8_t
         ready.store(k + 1, std::memory_order_release);
     }
     if (threads == 0) hand_over();
+    guard.ok = true;
     return CLF_OK;
 }
 
