@@ -5,10 +5,11 @@
 
 A step is one search of the whole batch (default: 1,000 synthetic events of
 uneven size, Poisson(7.6) vertices x Poisson(20) tracks, 1 % noise) with
-the events already resident in HBM: one launch of the persistent search
-kernel, which also packs the tracks and, from its last workgroup, clears
-its queue words for the next run (``--graph``: one replay of a hipGraph
-holding that launch instead).  ``value`` is
+the events already resident in HBM: one replay of a hipGraph holding the
+persistent search launch, which also packs the tracks and, from its last
+workgroup, clears its queue words for the next run (``--no-graph``: the
+launch made directly; the line reports the direct rate beside ``value``
+either way).  ``value`` is
 events per second over exactly ``--steps`` steps, bracketed by stream
 synchronisation.  Beside it the line carries the plain one-event-per-launch
 path on the same events, the roofline of the search kernel (algorithmic
@@ -30,6 +31,14 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
+
+# The HIP runtime's graph packet capture (dispatch packets built when a
+# graph is instantiated) made every replay of the search launch run 4-5 %
+# longer than the same launch made directly: 1.29M against 1.355M events/s
+# on batch1000, same box.  With it off a replay runs as fast as a direct
+# launch (1.35M).  The runtime reads it when it starts, so it is set here,
+# before the library's first HIP call; a value in the environment wins.
+os.environ.setdefault("DEBUG_CLR_GRAPH_PACKET_CAPTURE", "0")
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 TRACK_BYTES = 100
@@ -204,12 +213,10 @@ def main(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    # direct launches measured 4.4 % faster than graph replays of the same
-    # one-launch step on batch1000 (three same-box pairs; stress1000 equal)
     ap.add_argument("--graph", action="store_true",
-                    help="replay a hipGraph of the run instead of launching it directly")
+                    help="replay a hipGraph of the run (the default)")
     ap.add_argument("--no-graph", action="store_true",
-                    help="launch the run directly (the default)")
+                    help="launch the run directly instead of replaying a graph")
     ap.add_argument("--probe-copy", type=int, default=0, metavar="BYTES",
                     help="only run the calibration copy kernel over BYTES and exit")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving runs")
@@ -263,7 +270,7 @@ def main(argv=None):
         cfg["grid_size"] = a.grid
     if a.lds_hits:
         cfg["lds_hits"] = a.lds_hits
-    cfg["use_graph"] = 1 if a.graph and not a.no_graph else 0
+    cfg["use_graph"] = 0 if a.no_graph else 1
     ctx.set_config(**cfg)
     cfg = ctx.config()
     batch = ctx.batch(blobs)
@@ -323,6 +330,24 @@ def main(argv=None):
     ms_per_step = (t_end - t_start) * 1e3 / a.steps
     value = n_events / (ms_per_step / 1e3)
     log(f"batched: {ms_per_step:.3f} ms/step, {value:.0f} events/s")
+
+    # the same steps in the other launch form, timed the same way after it
+    other = 1 - cfg["use_graph"]
+    ctx.set_config(use_graph=other)
+    for _ in range(a.warmup):
+        batch.run(MODE_BATCHED)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        batch.run(MODE_BATCHED)
+    ctx.synchronize()
+    other_ms = (time.perf_counter() - t0) * 1e3 / a.steps
+    ctx.set_config(use_graph=cfg["use_graph"])
+    launch_other = {"form": "hipGraph replay" if other else "direct launch",
+                    "ms_per_step": round(other_ms, 4),
+                    "events_per_s": round(n_events / (other_ms / 1e3), 1),
+                    "graph_packet_capture": os.environ.get("DEBUG_CLR_GRAPH_PACKET_CAPTURE")}
+    log(f"{launch_other['form']}: {other_ms:.3f} ms/step")
 
     # device-side timings with hipEvents on the library's stream
     dev = batch.time(MODE_BATCHED, warmup=1, reps=a.steps)
@@ -424,13 +449,16 @@ def main(argv=None):
             "hits": sum(int.from_bytes(b[4:8], "little", signed=True) for b in blobs),
             "tracks": n_tracks,
             "path": ("batched persistent kernel, " +
-                     ("hipGraph replay" if cfg["use_graph"] else "direct launch") +
+                     ("hipGraph replay (runtime graph packet capture " +
+                      ("off" if os.environ.get("DEBUG_CLR_GRAPH_PACKET_CAPTURE") == "0" else "on") + ")"
+                      if cfg["use_graph"] else "direct launch") +
                      ", inputs resident in HBM"),
             "block_size": cfg["block_size"],
             "grid_size": cfg["grid_size"],
             "lds_hits": cfg["lds_hits"],
             "flags_seen": flags_any,
         },
+        "launch_other": launch_other,
         "device_ms_per_step": round(dev["ms_run"], 4),
         "step_spread": spread,
         "plain": plain,
