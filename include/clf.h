@@ -74,8 +74,12 @@ typedef struct clf_launch_config {
     int32_t grain;      /* consecutive queue items (events, in cost order)
                            a workgroup takes per dequeue, 1..64              */
     int32_t use_graph;  /* batched mode: capture the run in a hipGraph and
-                           replay it; 0 (the default): launch it directly,
-                           which measured faster for the one-launch run    */
+                           replay it; 0 (the default): launch it directly.
+                           Replays run as fast as direct launches with the
+                           HIP runtime's graph packet capture off
+                           (DEBUG_CLR_GRAPH_PACKET_CAPTURE=0 in the
+                           environment before the first HIP call); with it
+                           on they measured 4-5 % slower                   */
     int32_t lds_hits;   /* hits per LDS sensor slot, 256 or 512, or 0: with
                            the launch (256 for a set block_size); a sensor
                            with more hits is read from HBM                   */
