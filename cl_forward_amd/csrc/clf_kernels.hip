@@ -1845,7 +1845,7 @@ __device__ __forceinline__ void process_event(const SearchArgs& a, int e, Shared
 // count (fenced), so the last one sees the allocator's final value.
 template <int B, int C>
 __device__ __forceinline__ void run_end(const SearchArgs& a, Shared<B, C>& sh) {
-    if (!a.done) return;
+    if (!a.done || a.done_total <= 0) return;  // no end step: the caller clears
     if (threadIdx.x == 0) {
         __threadfence();
         sh.item = atomicAdd(a.done, 1u) + 1u == (unsigned int)a.done_total ? 1 : 0;
