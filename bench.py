@@ -371,8 +371,8 @@ def main(argv=None):
                  "tracks_identical": p_outs == outs}
         log(f"plain: {pt['ms_run']:.3f} ms/step; identical tracks: {plain['tracks_identical']}")
 
-    # the boundary's own call (host buffers in, host buffers out): upload,
-    # batched run and read-back of the whole batch, as clf_search does
+    # the Python binding's call (host buffers in, host buffers out): upload,
+    # batched run and read-back of the whole batch, per-event bytes out
     e2e = None
     if not a.no_plain:
         ctx.search(blobs[:8], MODE_BATCHED)  # warm-up of the allocation path
@@ -383,7 +383,7 @@ def main(argv=None):
         dt = (time.perf_counter() - t0) / reps
         e2e = {"ms_per_call": round(dt * 1e3, 3), "events_per_s": round(n_events / dt, 1),
                "tracks_identical": e_outs == outs,
-               "what": "clf_search: allocate, upload, one batched run, read back (host in, host out)"}
+               "what": "Context.search through the Python binding: upload, one batched run, read back, one bytes object per event (host in, host out)"}
         log(f"end to end: {dt * 1e3:.3f} ms per call")
 
     # the same call from C++, as the Gaudi offload handler makes it:

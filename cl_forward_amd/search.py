@@ -40,19 +40,43 @@ def _u8(buf):
 
 
 def _event_arrays(events):
-    """Keeps ctypes copies alive and returns (holders, pointer array, size array)."""
+    """Returns (holders, pointer array, size array) for read-only C calls.
+    A ``bytes`` event is passed by the address of its own buffer (no copy;
+    the library only reads events); any other buffer is copied into a
+    ``bytes`` first.  ``holders`` keeps those objects alive for the call."""
     n = len(events)
     holders = []
     ptrs = (ctypes.POINTER(ctypes.c_uint8) * max(n, 1))()
     sizes = (ctypes.c_size_t * max(n, 1))()
     for i, ev in enumerate(events):
-        b = bytes(ev) if ev is not None else b""
+        b = ev if isinstance(ev, bytes) else (bytes(ev) if ev is not None else b"")
         if b:
-            h = (ctypes.c_uint8 * len(b)).from_buffer_copy(b)
-            holders.append(h)
-            ptrs[i] = _u8(h)
+            holders.append(b)
+            ptrs[i] = ctypes.cast(ctypes.c_char_p(b), ctypes.POINTER(ctypes.c_uint8))
         sizes[i] = len(b)
     return holders, ptrs, sizes
+
+
+def _unpack(r):
+    """(list of per-event output blobs, list of stats dicts) of a clf_result,
+    whose arrays live as long as the batch that filled it."""
+    n = r.n_events
+    offs = [int(r.track_offset[i]) for i in range(n + 1)]
+    total = offs[-1] if n else 0
+    stats = []
+    for i in range(n):
+        s = r.stats[i]
+        stats.append({k: int(getattr(s, k)) for k, _ in EventStats._fields_})
+    if not total:
+        return [b"" for _ in range(n)], stats
+    # a view of the batch's host copy of the packed records; each event's
+    # records are copied out of it once
+    addr = ctypes.cast(r.tracks, ctypes.c_void_p).value
+    raw = memoryview((ctypes.c_uint8 * (total * TRACK_BYTES)).from_address(addr)).cast("B")
+    outs = [bytes(raw[offs[i] * TRACK_BYTES: (offs[i] + stats[i]["n_tracks"]) * TRACK_BYTES])
+            for i in range(n)]
+    raw.release()
+    return outs, stats
 
 
 class Context:
@@ -127,10 +151,19 @@ class Context:
         return Batch(self, events)
 
     def search(self, events, mode: int = MODE_BATCHED):
-        """Upload, run and read back.  Returns (list of output blobs, list of stats)."""
-        with self.batch(events) as b:
-            b.run(mode)
-            return b.results()
+        """clf_search: upload, run and read back in stream order (no graph;
+        a batch that runs once does not pay for a capture).  Returns (list of
+        output blobs, list of stats)."""
+        holders, ptrs, sizes = _event_arrays(events)
+        r = Result()
+        h = ctypes.c_void_p()
+        _lib.check(self._L.clf_search(self._h, int(mode), len(events), ptrs, sizes, ctypes.byref(r),
+                                      ctypes.byref(h)), "clf_search")
+        del holders
+        try:
+            return _unpack(r)
+        finally:
+            self._L.clf_batch_destroy(h)
 
     def search_noalloc(self, events, mode: int = MODE_BATCHED, capacity: int | None = None):
         """clf_search_noalloc: the search without any device allocation (the
@@ -215,17 +248,7 @@ class Batch:
         r = Result()
         _lib.check(self._L.clf_batch_results(self.ctx.handle, self._h, ctypes.byref(r)),
                    "clf_batch_results")
-        n = r.n_events
-        offs = [int(r.track_offset[i]) for i in range(n + 1)]
-        total = offs[-1] if n else 0
-        raw = ctypes.string_at(r.tracks, total * TRACK_BYTES) if total else b""
-        stats = []
-        for i in range(n):
-            s = r.stats[i]
-            stats.append({k: int(getattr(s, k)) for k, _ in EventStats._fields_})
-        outs = [raw[offs[i] * TRACK_BYTES: (offs[i] + stats[i]["n_tracks"]) * TRACK_BYTES]
-                for i in range(n)]
-        return outs, stats
+        return _unpack(r)
 
     PROFILE_FIELDS = ("prologue", "stage", "forward", "gather", "seed_setup", "stage_wait",
                       "seed_eval", "seed_emit", "tail", "total", "fwd_entries", "visits", "pairs",
