@@ -373,3 +373,29 @@ def test_autotune_keeps_grain_one_within_margin():
     assert pick_best([one, near]) is one
     assert pick_best([one, near, far]) is far
     assert pick_best([one, row(1.2, 2)]) is one
+
+
+def test_event_arrays_point_at_the_events():
+    """The binding hands `bytes` events to the library by their own buffer
+    (no copy) and copies other buffers once; every pointer reads back the
+    event's bytes, and empty or missing events pass a null pointer and
+    size 0."""
+    import ctypes
+
+    from cl_forward_amd.search import _event_arrays
+
+    blobs = [generator.single_event().pack(),
+             raw_blob(2, 3, [0, 22], [0, 2], [2, 1], [1.0, 2.0, 3.0], [0.0] * 3, [0.0, 0.0, 22.0])]
+    events = [blobs[0], bytearray(blobs[1]), memoryview(blobs[0]), b"", None]
+    holders, ptrs, sizes = _event_arrays(events)
+    want = [blobs[0], blobs[1], blobs[0], b"", b""]
+    assert [sizes[i] for i in range(len(events))] == [len(w) for w in want]
+    for i, w in enumerate(want):
+        if w:
+            assert ctypes.string_at(ptrs[i], sizes[i]) == w
+        else:
+            assert not ptrs[i]
+    # the bytes event itself is what the pointer addresses
+    assert ctypes.cast(ptrs[0], ctypes.c_void_p).value == \
+        ctypes.cast(ctypes.c_char_p(blobs[0]), ctypes.c_void_p).value
+    assert holders[0] is blobs[0]
