@@ -181,9 +181,11 @@ class Context:
                                               int(capacity), offs, stats),
                    "clf_search_noalloc")
         del holders
-        raw = bytes(tracks)
+        raw = memoryview(tracks).cast("B")
         st = [{k: int(getattr(stats[i], k)) for k, _ in EventStats._fields_} for i in range(n)]
-        outs = [raw[offs[i] * TRACK_BYTES: (offs[i] + st[i]["n_tracks"]) * TRACK_BYTES] for i in range(n)]
+        outs = [bytes(raw[offs[i] * TRACK_BYTES: (offs[i] + st[i]["n_tracks"]) * TRACK_BYTES])
+                for i in range(n)]
+        raw.release()
         return outs, st
 
     def fill_candidates(self, events, search_path: bool = False):
